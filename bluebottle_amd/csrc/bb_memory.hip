@@ -1,0 +1,430 @@
+/*
+ * bb_memory.hip -- device context, allocation and host<->device transfers.
+ *
+ * Replaces src/cuda_bluebottle.cu:36-1547 (cuda_dom_malloc / push / pull /
+ * free) and src/cuda_particle.cu:28-354 (cuda_part_malloc / push / pull /
+ * free).  As in the reference, these allocate the `real **_x` device-pointer
+ * tables (one entry per subdomain; this library drives one) and the device
+ * arrays themselves; host arrays stay owned by the driver.  All work runs
+ * on one library stream at default priority; transfers to the host
+ * synchronise it.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+#include "bb_device.h"
+
+BBContext g_bb;
+
+void bb_fatal(const char *what, hipError_t err, const char *file, int line)
+{
+  fprintf(stderr, "bluebottle_hip: %s failed at %s:%d: %s\n", what, file, line,
+    hipGetErrorString(err));
+  fflush(stderr);
+  exit(EXIT_FAILURE);
+}
+
+BBGeom bb_geom(void)
+{
+  const dom_struct *d = &dom[0];
+  BBGeom g;
+  g.xn = d->xn; g.yn = d->yn; g.zn = d->zn;
+  g.cs1 = d->Gcc.s1b; g.cs2 = d->Gcc.s2b;
+  g.us1 = d->Gfx.s1b; g.us2 = d->Gfx.s2b;
+  g.vs1 = d->Gfy.s1b; g.vs2 = d->Gfy.s2b;
+  g.ws1 = d->Gfz.s1b; g.ws2 = d->Gfz.s2b;
+  g.ccount = d->Gcc.s3b; g.ucount = d->Gfx.s3b;
+  g.vcount = d->Gfy.s3b; g.wcount = d->Gfz.s3b;
+  g.dx = d->dx; g.dy = d->dy; g.dz = d->dz;
+  g.xs = d->xs; g.ys = d->ys; g.zs = d->zs;
+  g.xe = d->xe; g.ye = d->ye; g.ze = d->ze;
+  g.xl = d->xl; g.yl = d->yl; g.zl = d->zl;
+  return g;
+}
+
+void bb_ensure_context(void)
+{
+  if(g_bb.ready) return;
+  int ndev = 0;
+  BB_CHECK(hipGetDeviceCount(&ndev));
+  if(ndev <= 0) {
+    fprintf(stderr, "bluebottle_hip: no HIP device visible\n");
+    exit(EXIT_FAILURE);
+  }
+  g_bb.device = dev_start < ndev ? dev_start : 0;
+  BB_CHECK(hipSetDevice(g_bb.device));
+  BB_CHECK(hipStreamCreateWithFlags(&g_bb.stream, hipStreamNonBlocking));
+  g_bb.exit_on_fail = 1;
+  g_bb.ready = 1;
+}
+
+static void *dmalloc(size_t bytes)
+{
+  void *ptr = NULL;
+  BB_CHECK(hipMalloc(&ptr, bytes ? bytes : 8));
+  gpumem += (long) bytes;
+  return ptr;
+}
+
+static void dfree(void *ptr)
+{
+  if(ptr) BB_CHECK(hipFree(ptr));
+}
+
+void bb_ensure_workspace(void)
+{
+  bb_ensure_context();
+  BBWorkspace *ws = &g_bb.ws;
+  size_t n = (size_t) dom[0].Gcc.s3;
+  if(ws->n == n && ws->x) return;
+  bb_free_workspace();
+  ws->n = n;
+  ws->nblk = 4096;
+  ws->x = (double *) dmalloc(n * sizeof(double));
+  ws->r = (double *) dmalloc(n * sizeof(double));
+  ws->rs = (double *) dmalloc(n * sizeof(double));
+  ws->p = (double *) dmalloc(n * sizeof(double));
+  ws->v = (double *) dmalloc(n * sizeof(double));
+  ws->t = (double *) dmalloc(n * sizeof(double));
+  ws->b = (double *) dmalloc(n * sizeof(double));
+  ws->xf = (float *) dmalloc(n * sizeof(float));
+  ws->rf = (float *) dmalloc(n * sizeof(float));
+  ws->rsf = (float *) dmalloc(n * sizeof(float));
+  ws->pf = (float *) dmalloc(n * sizeof(float));
+  ws->vf = (float *) dmalloc(n * sizeof(float));
+  ws->tf = (float *) dmalloc(n * sizeof(float));
+  ws->bf = (float *) dmalloc(n * sizeof(float));
+  ws->flags = (uint8_t *) dmalloc(n);
+  ws->partials = (double *) dmalloc((size_t) ws->nblk * 4 * sizeof(double));
+  ws->sc = (KrylovScalars *) dmalloc(sizeof(KrylovScalars));
+  ws->red = (double *) dmalloc(64 * sizeof(double));
+  BB_CHECK(hipHostMalloc((void **) &ws->sc_host, sizeof(KrylovScalars),
+    hipHostMallocDefault));
+  BB_CHECK(hipHostMalloc((void **) &ws->red_host, 64 * sizeof(double),
+    hipHostMallocDefault));
+}
+
+void bb_free_workspace(void)
+{
+  BBWorkspace *ws = &g_bb.ws;
+  void *ptrs[] = {ws->x, ws->r, ws->rs, ws->p, ws->v, ws->t, ws->b, ws->xf,
+    ws->rf, ws->rsf, ws->pf, ws->vf, ws->tf, ws->bf, ws->flags, ws->partials, ws->sc,
+    ws->red};
+  for(size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);
+  if(ws->sc_host) BB_CHECK(hipHostFree(ws->sc_host));
+  if(ws->red_host) BB_CHECK(hipHostFree(ws->red_host));
+  memset(ws, 0, sizeof(*ws));
+}
+
+/* ---------------------------------------------------------------------
+ * cuda_dom_*: src/cuda_bluebottle.cu:36-1547
+ * ------------------------------------------------------------------- */
+struct FieldDesc { real ***dev; real **host; int grid; };
+
+static int grid_count(int grid)
+{
+  switch(grid) {
+    case 0: return dom[0].Gcc.s3b;
+    case 1: return dom[0].Gfx.s3b;
+    case 2: return dom[0].Gfy.s3b;
+    default: return dom[0].Gfz.s3b;
+  }
+}
+
+static std::vector<FieldDesc> dom_fields(void)
+{
+  std::vector<FieldDesc> f = {
+    {&_p0, &p0, 0}, {&_p, &p, 0}, {&_phi, &phi, 0},
+    {&_u, &u, 1}, {&_u0, &u0, 1}, {&_v, &v, 2}, {&_v0, &v0, 2},
+    {&_w, &w, 3}, {&_w0, &w0, 3},
+    {&_f_x, &f_x, 1}, {&_f_y, &f_y, 2}, {&_f_z, &f_z, 3},
+    {&_diff0_u, &diff0_u, 1}, {&_diff0_v, &diff0_v, 2}, {&_diff0_w, &diff0_w, 3},
+    {&_conv0_u, &conv0_u, 1}, {&_conv0_v, &conv0_v, 2}, {&_conv0_w, &conv0_w, 3},
+    {&_diff_u, &diff_u, 1}, {&_diff_v, &diff_v, 2}, {&_diff_w, &diff_w, 3},
+    {&_conv_u, &conv_u, 1}, {&_conv_v, &conv_v, 2}, {&_conv_w, &conv_w, 3},
+    {&_u_star, &u_star, 1}, {&_v_star, &v_star, 2}, {&_w_star, &w_star, 3},
+  };
+  return f;
+}
+
+extern "C" void cuda_dom_malloc(void)
+{
+  bb_ensure_context();
+  _dom = (dom_struct **) malloc(nsubdom * sizeof(dom_struct *));
+  _dom[0] = (dom_struct *) dmalloc(sizeof(dom_struct));
+  for(auto &f : dom_fields()) {
+    *f.dev = (real **) malloc(nsubdom * sizeof(real *));
+    (*f.dev)[0] = (real *) dmalloc((size_t) grid_count(f.grid) * sizeof(real));
+    BB_CHECK(hipMemsetAsync((*f.dev)[0], 0,
+      (size_t) grid_count(f.grid) * sizeof(real), g_bb.stream));
+  }
+  _rhs_p = (real **) malloc(nsubdom * sizeof(real *));
+  _rhs_p[0] = (real *) dmalloc((size_t) dom[0].Gcc.s3 * sizeof(real));
+  BB_CHECK(hipMemsetAsync(_rhs_p[0], 0, (size_t) dom[0].Gcc.s3 * sizeof(real),
+    g_bb.stream));
+  bb_ensure_workspace();
+  BB_CHECK(hipStreamSynchronize(g_bb.stream));
+}
+
+extern "C" void cuda_dom_push(void)
+{
+  bb_ensure_context();
+  BB_CHECK(hipMemcpy(_dom[0], &dom[0], sizeof(dom_struct), hipMemcpyHostToDevice));
+  for(auto &f : dom_fields()) {
+    if(*f.host == NULL) continue;
+    BB_CHECK(hipMemcpyAsync((*f.dev)[0], *f.host,
+      (size_t) grid_count(f.grid) * sizeof(real), hipMemcpyHostToDevice,
+      g_bb.stream));
+  }
+  BB_CHECK(hipStreamSynchronize(g_bb.stream));
+}
+
+extern "C" void cuda_dom_pull(void)
+{
+  bb_ensure_context();
+  for(auto &f : dom_fields()) {
+    if(*f.host == NULL) continue;
+    BB_CHECK(hipMemcpyAsync(*f.host, (*f.dev)[0],
+      (size_t) grid_count(f.grid) * sizeof(real), hipMemcpyDeviceToHost,
+      g_bb.stream));
+  }
+  BB_CHECK(hipStreamSynchronize(g_bb.stream));
+}
+
+extern "C" void cuda_dom_free(void)
+{
+  if(!g_bb.ready) return;
+  BB_CHECK(hipStreamSynchronize(g_bb.stream));
+  for(auto &f : dom_fields()) {
+    if(*f.dev) { dfree((*f.dev)[0]); free(*f.dev); *f.dev = NULL; }
+  }
+  if(_rhs_p) { dfree(_rhs_p[0]); free(_rhs_p); _rhs_p = NULL; }
+  if(_dom) { dfree(_dom[0]); free(_dom); _dom = NULL; }
+  bb_free_workspace();
+}
+
+/* ---------------------------------------------------------------------
+ * cuda_part_*: src/cuda_particle.cu:28-354
+ * ------------------------------------------------------------------- */
+struct CoeffDesc { real ***dev; real **host; };
+
+static std::vector<CoeffDesc> coeff_fields(void)
+{
+  std::vector<CoeffDesc> c = {
+    {&_pnm_re, &pnm_re}, {&_pnm_im, &pnm_im}, {&_phinm_re, &phinm_re},
+    {&_phinm_im, &phinm_im}, {&_chinm_re, &chinm_re}, {&_chinm_im, &chinm_im},
+    {&_pnm_re0, &pnm_re0}, {&_pnm_im0, &pnm_im0}, {&_phinm_re0, &phinm_re0},
+    {&_phinm_im0, &phinm_im0}, {&_chinm_re0, &chinm_re0},
+    {&_chinm_im0, &chinm_im0},
+    {&_pnm_re00, &pnm_re00}, {&_pnm_im00, &pnm_im00},
+    {&_phinm_re00, &phinm_re00}, {&_phinm_im00, &phinm_im00},
+    {&_chinm_re00, &chinm_re00}, {&_chinm_im00, &chinm_im00},
+  };
+  return c;
+}
+
+struct IntDesc { int ***dev; int **host; int grid; };
+
+static std::vector<IntDesc> int_fields(void)
+{
+  std::vector<IntDesc> f = {
+    {&_phase, &phase, 0}, {&_phase_shell, &phase_shell, 0},
+    {&_flag_u, &flag_u, 1}, {&_flag_v, &flag_v, 2}, {&_flag_w, &flag_w, 3},
+  };
+  return f;
+}
+
+static size_t coeff_bytes(void)
+{
+  return (size_t) coeff_stride * (size_t) nparts * sizeof(real);
+}
+
+extern "C" void cuda_part_malloc(void)
+{
+  bb_ensure_context();
+  _parts = (part_struct **) malloc(nsubdom * sizeof(part_struct *));
+  _parts[0] = (part_struct *) dmalloc((size_t) nparts * sizeof(part_struct));
+  for(auto &c : coeff_fields()) {
+    *c.dev = (real **) malloc(nsubdom * sizeof(real *));
+    (*c.dev)[0] = (real *) dmalloc(coeff_bytes());
+    BB_CHECK(hipMemsetAsync((*c.dev)[0], 0, coeff_bytes() ? coeff_bytes() : 8,
+      g_bb.stream));
+  }
+  for(auto &f : int_fields()) {
+    *f.dev = (int **) malloc(nsubdom * sizeof(int *));
+    (*f.dev)[0] = (int *) dmalloc((size_t) grid_count(f.grid) * sizeof(int));
+  }
+  _binDom = (dom_struct *) dmalloc(sizeof(dom_struct));
+  bb_ensure_particle_workspace();
+  BB_CHECK(hipStreamSynchronize(g_bb.stream));
+}
+
+extern "C" void cuda_part_push(void)
+{
+  bb_ensure_context();
+  if(nparts > 0)
+    BB_CHECK(hipMemcpyAsync(_parts[0], parts, (size_t) nparts * sizeof(part_struct),
+      hipMemcpyHostToDevice, g_bb.stream));
+  if(coeff_bytes() > 0)
+    for(auto &c : coeff_fields())
+      if(*c.host)
+        BB_CHECK(hipMemcpyAsync((*c.dev)[0], *c.host, coeff_bytes(),
+          hipMemcpyHostToDevice, g_bb.stream));
+  for(auto &f : int_fields())
+    if(*f.host)
+      BB_CHECK(hipMemcpyAsync((*f.dev)[0], *f.host,
+        (size_t) grid_count(f.grid) * sizeof(int), hipMemcpyHostToDevice,
+        g_bb.stream));
+  BB_CHECK(hipMemcpyAsync(_binDom, &binDom, sizeof(dom_struct),
+    hipMemcpyHostToDevice, g_bb.stream));
+  BB_CHECK(hipStreamSynchronize(g_bb.stream));
+}
+
+extern "C" void cuda_part_pull(void)
+{
+  bb_ensure_context();
+  if(nparts > 0)
+    BB_CHECK(hipMemcpyAsync(parts, _parts[0], (size_t) nparts * sizeof(part_struct),
+      hipMemcpyDeviceToHost, g_bb.stream));
+  if(coeff_bytes() > 0)
+    for(auto &c : coeff_fields())
+      if(*c.host)
+        BB_CHECK(hipMemcpyAsync(*c.host, (*c.dev)[0], coeff_bytes(),
+          hipMemcpyDeviceToHost, g_bb.stream));
+  for(auto &f : int_fields())
+    if(*f.host)
+      BB_CHECK(hipMemcpyAsync(*f.host, (*f.dev)[0],
+        (size_t) grid_count(f.grid) * sizeof(int), hipMemcpyDeviceToHost,
+        g_bb.stream));
+  BB_CHECK(hipStreamSynchronize(g_bb.stream));
+}
+
+extern "C" void cuda_part_free(void)
+{
+  if(!g_bb.ready) return;
+  BB_CHECK(hipStreamSynchronize(g_bb.stream));
+  if(_parts) { dfree(_parts[0]); free(_parts); _parts = NULL; }
+  for(auto &c : coeff_fields())
+    if(*c.dev) { dfree((*c.dev)[0]); free(*c.dev); *c.dev = NULL; }
+  for(auto &f : int_fields())
+    if(*f.dev) { dfree((*f.dev)[0]); free(*f.dev); *f.dev = NULL; }
+  dfree(_binDom); _binDom = NULL;
+  dfree(g_bb.face_list); dfree(g_bb.face_count);
+  dfree(g_bb.bin_ind); dfree(g_bb.bin_bin); dfree(g_bb.bin_start);
+  dfree(g_bb.bin_end); dfree(g_bb.quad); dfree(g_bb.coeff_err);
+  dfree(g_bb.node_t); dfree(g_bb.node_p);
+  if(g_bb.host_err) BB_CHECK(hipHostFree(g_bb.host_err));
+  g_bb.face_list = NULL; g_bb.face_count = NULL; g_bb.face_cap = 0;
+  g_bb.bin_ind = g_bb.bin_bin = g_bb.bin_start = g_bb.bin_end = NULL;
+  g_bb.bin_cap = g_bb.nbins_cap = 0;
+  g_bb.quad = g_bb.coeff_err = g_bb.node_t = g_bb.node_p = NULL;
+  g_bb.host_err = NULL;
+  g_bb.part_cap = 0;
+}
+
+/* ---------------------------------------------------------------------
+ * Library extras (no counterpart in the reference).
+ * ------------------------------------------------------------------- */
+extern "C" void bb_set_solver_mode(int mode) { g_bb.solver_mode = mode; }
+extern "C" int bb_get_solver_mode(void) { return g_bb.solver_mode; }
+extern "C" void bb_get_solve_stats(bb_solve_stats *out) { *out = g_bb.stats; }
+
+/* 1 (default): a Poisson solve that does not converge prints the
+ * reference's message and exits; 0: it returns and reports converged = 0
+ * in the solve statistics. */
+extern "C" void bb_set_exit_on_nonconvergence(int on)
+{
+  bb_ensure_context();
+  g_bb.exit_on_fail = on;
+}
+
+static char g_devname[256];
+extern "C" const char *bb_device_name(void)
+{
+  bb_ensure_context();
+  hipDeviceProp_t prop;
+  BB_CHECK(hipGetDeviceProperties(&prop, g_bb.device));
+  snprintf(g_devname, sizeof(g_devname), "%s (%s, %d CUs)", prop.name,
+    prop.gcnArchName, prop.multiProcessorCount);
+  return g_devname;
+}
+
+extern "C" void bb_synchronize(void)
+{
+  bb_ensure_context();
+  BB_CHECK(hipStreamSynchronize(g_bb.stream));
+}
+
+extern "C" void *bb_stream(void)
+{
+  bb_ensure_context();
+  return (void *) g_bb.stream;
+}
+
+/* ---- per-kernel event timing ---- */
+struct ProfRec { int kind; hipEvent_t a, b; };
+static std::vector<ProfRec> g_prof_pool;
+static size_t g_prof_used = 0;
+static int g_prof_on = 0;
+static int g_prof_open[BB_K_NKINDS];
+static double g_prof_ms[BB_K_NKINDS];
+static long g_prof_cnt[BB_K_NKINDS];
+
+extern "C" void bb_prof_enable(int on)
+{
+  bb_ensure_context();
+  g_prof_on = on;
+  if(on && g_prof_pool.empty()) {
+    g_prof_pool.resize(1 << 15);
+    for(auto &r : g_prof_pool) {
+      BB_CHECK(hipEventCreate(&r.a));
+      BB_CHECK(hipEventCreate(&r.b));
+    }
+  }
+}
+
+static void prof_drain(void)
+{
+  if(g_prof_used == 0) return;
+  BB_CHECK(hipStreamSynchronize(g_bb.stream));
+  for(size_t i = 0; i < g_prof_used; i++) {
+    float ms = 0.f;
+    BB_CHECK(hipEventElapsedTime(&ms, g_prof_pool[i].a, g_prof_pool[i].b));
+    g_prof_ms[g_prof_pool[i].kind] += ms;
+    g_prof_cnt[g_prof_pool[i].kind] += 1;
+  }
+  g_prof_used = 0;
+}
+
+void bb_prof_begin(int kind)
+{
+  if(!g_prof_on) return;
+  if(g_prof_used >= g_prof_pool.size()) prof_drain();
+  g_prof_open[kind] = (int) g_prof_used;
+  g_prof_pool[g_prof_used].kind = kind;
+  BB_CHECK(hipEventRecord(g_prof_pool[g_prof_used].a, g_bb.stream));
+  g_prof_used++;
+}
+
+void bb_prof_end(int kind)
+{
+  if(!g_prof_on) return;
+  BB_CHECK(hipEventRecord(g_prof_pool[g_prof_open[kind]].b, g_bb.stream));
+}
+
+/* Total milliseconds and launch count recorded for `kind` since the last
+ * reset. */
+extern "C" void bb_prof_read(int kind, double *ms, long *count)
+{
+  prof_drain();
+  *ms = (kind >= 0 && kind < BB_K_NKINDS) ? g_prof_ms[kind] : 0.;
+  *count = (kind >= 0 && kind < BB_K_NKINDS) ? g_prof_cnt[kind] : 0;
+}
+
+extern "C" void bb_prof_reset(void)
+{
+  prof_drain();
+  memset(g_prof_ms, 0, sizeof(g_prof_ms));
+  memset(g_prof_cnt, 0, sizeof(g_prof_cnt));
+}

@@ -1,0 +1,823 @@
+/*
+ * bb_poisson.hip -- the pressure-Poisson solve, cuda_PP_bicgstab
+ * (src/cuda_bicgstab.cu:715-1073).
+ *
+ * The reference assembles a 13-diagonal CUSP DIA matrix every call
+ * (13 x 8 B = 104 B/cell, src/bicgstab_kernel.cu:1654-1856) and runs
+ * cusp::krylov::bicgstab with a diagonal preconditioner, paying a
+ * host-synchronous reduction per dot product.  Here the operator is
+ * matrix-free: one byte per cell packs the six |face flag| bits and the
+ * solid bit, and each off-diagonal is that bit times 1/h^2 -- exactly the
+ * values the DIA assembly stores, including the periodic wrap diagonals
+ * (offsets -s3+s2, -s2+s1, -s1+1, s1-1, s2-s1, s3-s2 of
+ * src/cuda_bicgstab.cu:735-747 become index wraps) and the identity rows
+ * of solid cells.  Jacobi scaling uses a 128-entry table of 1/diag
+ * computed on the host with the reference's diagonal arithmetic.
+ *
+ * Iteration structure (per BiCGSTAB step, CUSP semantics):
+ *   K1  v = A (M p),              partial <r*, v>
+ *   F1  alpha = rho / <r*, v>
+ *   K2  s = r - alpha v (on the fly), t = A (M s),  partials <t,s>,<t,t>,<s,s>
+ *   F2  ||s|| test, omega = <t,s>/<t,t>
+ *   K3  x += alpha M p + omega M s,  r = s - omega t,  partials <r*,r>,<r,r>
+ *   F3  beta, rho, ||r|| test, iteration count
+ *   K4  p = r + beta (p - omega v)
+ * Dots are reduced deterministically (per-block partials, then one
+ * finishing block); the scalars stay on the device and the host polls the
+ * `done` flag every few iterations.
+ *
+ * Mixed mode (bb_set_solver_mode(BB_SOLVER_MIXED)): FP64 iterative
+ * refinement around the same BiCGSTAB run in FP32 (FP32 vectors, FP64
+ * accumulation of FP32 products in every dot).  Convergence is judged on
+ * the true FP64 residual ||b - A x|| <= pp_residual * ||b||.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include "bb_device.h"
+
+#define PX 64
+#define PY 4
+#define PZ 16
+
+struct PPGeom {
+  int nx, ny, nz;
+  int tx, ty, tz;          /* tiles of PX x PY x PZ */
+  int perx, pery, perz;
+  double ddx, ddy, ddz;    /* 1/h^2 */
+  int nblk;
+};
+
+struct DinvTable { double d[128]; double g[128]; };
+
+static DinvTable make_dinv(double ddx, double ddy, double ddz)
+{
+  DinvTable t;
+  for(int f = 0; f < 128; f++) {
+    if(f & BB_SOLID) { t.d[f] = 1.; t.g[f] = 1.; continue; }
+    int fx = ((f & BB_FW) ? 1 : 0) + ((f & BB_FE) ? 1 : 0);
+    int fy = ((f & BB_FS) ? 1 : 0) + ((f & BB_FN) ? 1 : 0);
+    int fz = ((f & BB_FB) ? 1 : 0) + ((f & BB_FT) ? 1 : 0);
+    /* values[6] of src/bicgstab_kernel.cu:1710-1712 */
+    double dg = 0.;
+    dg -= (double) fx * ddx;
+    dg -= (double) fy * ddy;
+    dg -= (double) fz * ddz;
+    t.g[f] = dg;
+    t.d[f] = 1. / dg;
+  }
+  return t;
+}
+
+static PPGeom pp_geom(void)
+{
+  PPGeom g;
+  const dom_struct *d = &dom[0];
+  g.nx = d->xn; g.ny = d->yn; g.nz = d->zn;
+  g.tx = (g.nx + PX - 1) / PX;
+  g.ty = (g.ny + PY - 1) / PY;
+  g.tz = (g.nz + PZ - 1) / PZ;
+  g.perx = bc.pW == PERIODIC;
+  g.pery = bc.pS == PERIODIC;
+  g.perz = bc.pB == PERIODIC;
+  g.ddx = 1. / (d->dx * d->dx);
+  g.ddy = 1. / (d->dy * d->dy);
+  g.ddz = 1. / (d->dz * d->dz);
+  g.nblk = g.tx * g.ty * g.tz;
+  return g;
+}
+
+/* Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD
+ * works on a contiguous range of tiles (its own L2 then holds the y and z
+ * halos of neighbouring tiles). */
+__device__ inline int xcd_block(int nblk)
+{
+  int b = blockIdx.x;
+  if(nblk % 8 == 0) return (b % 8) * (nblk / 8) + b / 8;
+  return b;
+}
+
+struct TileIdx { int i, j, k0, k1; };
+
+__device__ inline TileIdx tile_of(const PPGeom &G, int lb)
+{
+  TileIdx t;
+  int tx = lb % G.tx;
+  int rest = lb / G.tx;
+  int ty = rest % G.ty;
+  int tz = rest / G.ty;
+  t.i = tx * PX + threadIdx.x;
+  t.j = ty * PY + threadIdx.y;
+  t.k0 = tz * PZ;
+  t.k1 = min(t.k0 + PZ, G.nz);
+  return t;
+}
+
+/* Neighbour offsets of cell (i,j,k) honouring the periodic wraps of the
+ * DIA matrix; non-periodic boundary neighbours carry a zero coefficient
+ * (their face flag is 0), so any in-range index serves. */
+struct Nbr { int w, e, s, n, b, t; };
+
+__device__ inline Nbr neighbours(const PPGeom &G, int i, int j, int k)
+{
+  const int s1 = G.nx, s2 = G.nx * G.ny;
+  Nbr o;
+  o.w = (i > 0) ? -1 : (G.perx ? G.nx - 1 : 0);
+  o.e = (i < G.nx - 1) ? 1 : (G.perx ? -(G.nx - 1) : 0);
+  o.s = (j > 0) ? -s1 : (G.pery ? (G.ny - 1) * s1 : 0);
+  o.n = (j < G.ny - 1) ? s1 : (G.pery ? -(G.ny - 1) * s1 : 0);
+  o.b = (k > 0) ? -s2 : (G.perz ? (G.nz - 1) * s2 : 0);
+  o.t = (k < G.nz - 1) ? s2 : (G.perz ? -(G.nz - 1) * s2 : 0);
+  return o;
+}
+
+/* (A y)_C for the row of cell C given y at the centre and six neighbours,
+ * in the DIA summation order (diagonals 0..12). */
+__device__ inline double apply_row(uint8_t f, const PPGeom &G, double yc,
+  double yw, double ye, double ys, double yn, double yb, double yt,
+  double dg)
+{
+  if(f & BB_SOLID) return yc;
+  double a = 0.;
+  a += (f & BB_FB) ? G.ddz * yb : 0.;
+  a += (f & BB_FS) ? G.ddy * ys : 0.;
+  a += (f & BB_FW) ? G.ddx * yw : 0.;
+  a += dg * yc;
+  a += (f & BB_FE) ? G.ddx * ye : 0.;
+  a += (f & BB_FN) ? G.ddy * yn : 0.;
+  a += (f & BB_FT) ? G.ddz * yt : 0.;
+  return a;
+}
+
+/* diag entry from the flag byte (the diagonal table sits right after the
+ * 128 reciprocals in LDS) */
+__device__ inline double diag_of(const double *dinv, uint8_t f)
+{
+  return dinv[128 + (f & 127)];
+}
+
+/* ---------------------------------------------------------------------
+ * RHS + flag packing + warm start (one pass over the fields).
+ *   rhs = rho_f/dt * div(u*)                (PP_rhs)
+ *   rhs *= (phase < 0 && phase_shell)       (part_BC_p's final mask; its
+ *                                            Lamb term is multiplied by 0)
+ *   flags, x = phi(interior), partial sum of rhs^2
+ * ------------------------------------------------------------------- */
+__global__ void __launch_bounds__(256)
+k_pp_setup(PPGeom G, BBGeom B, double rho_dt, int mask,
+  const double *__restrict__ us, const double *__restrict__ vs,
+  const double *__restrict__ ws, const double *__restrict__ phi,
+  const int *__restrict__ phase, const int *__restrict__ shell,
+  const int *__restrict__ fu, const int *__restrict__ fv,
+  const int *__restrict__ fw, double *__restrict__ rhs,
+  double *__restrict__ x, uint8_t *__restrict__ flags,
+  double *__restrict__ part)
+{
+  __shared__ double sh[4];
+  const int lb = xcd_block(G.nblk);
+  TileIdx T = tile_of(G, lb);
+  const double ddx = 1. / B.dx, ddy = 1. / B.dy, ddz = 1. / B.dz;
+  double s2 = 0.;
+  if(T.i < G.nx && T.j < G.ny) {
+    for(int k = T.k0; k < T.k1; k++) {
+      const int i = T.i + 1, j = T.j + 1, kk = k + 1;
+      const int U = i + j*B.us1 + kk*B.us2;
+      const int V = i + j*B.vs1 + kk*B.vs2;
+      const int W = i + j*B.ws1 + kk*B.ws2;
+      const int Cc = i + j*B.cs1 + kk*B.cs2;
+      double r = (us[U+1] - us[U]) * ddx;
+      r += (vs[V+B.vs1] - vs[V]) * ddy;
+      r += (ws[W+B.ws2] - ws[W]) * ddz;
+      r *= rho_dt;
+      int ph = phase[Cc];
+      if(mask) r = (double) (ph < 0 && shell[Cc]) * r;
+      uint8_t f = 0;
+      f |= fu[U] ? BB_FW : 0;
+      f |= fu[U+1] ? BB_FE : 0;
+      f |= fv[V] ? BB_FS : 0;
+      f |= fv[V+B.vs1] ? BB_FN : 0;
+      f |= fw[W] ? BB_FB : 0;
+      f |= fw[W+B.ws2] ? BB_FT : 0;
+      f |= (ph > -1) ? BB_SOLID : 0;
+      const int C = T.i + T.j*G.nx + k*G.nx*G.ny;
+      rhs[C] = r;
+      flags[C] = f;
+      x[C] = phi[Cc];
+      s2 += r * r;
+    }
+  }
+  double t = bb_block_sum(s2, sh);
+  if(threadIdx.x == 0 && threadIdx.y == 0) part[lb] = t;
+}
+
+/* ---------------------------------------------------------------------
+ * Finishing kernels (one block of 256): sum the per-block partials in a
+ * fixed order and advance the scalar state.
+ * ------------------------------------------------------------------- */
+__device__ inline void sum_partials(const double *part, int nblk, int nq,
+  double *out)
+{
+  __shared__ double sh[4];
+  for(int q = 0; q < nq; q++) {
+    double a = 0.;
+    for(int b = threadIdx.x; b < nblk; b += blockDim.x) a += part[b*nq + q];
+    out[q] = bb_block_sum(a, sh);
+  }
+}
+
+/* norm of the raw rhs; 1 when it is 0 (src/cuda_bicgstab.cu:996-1000) */
+__global__ void k_f_norm(int nblk, const double *part, KrylovScalars *sc)
+{
+  double o[1];
+  sum_partials(part, nblk, 1, o);
+  if(threadIdx.x == 0) {
+    double nrm = sqrt(o[0]);
+    if(nrm == 0) nrm = 1.;
+    sc->bnorm = nrm;
+  }
+}
+
+/* r0 = b - A x0, tolerance, first convergence test */
+__global__ void k_f_init(int nblk, const double *part, KrylovScalars *sc,
+  double rel_tol, int max_iter)
+{
+  double o[2];
+  sum_partials(part, nblk, 2, o);
+  if(threadIdx.x == 0) {
+    sc->tol = 0. + rel_tol * sqrt(o[0]);
+    sc->rho = o[1];
+    sc->resid = sqrt(o[1]);
+    sc->iter = 0;
+    sc->max_iter = max_iter;
+    sc->s_conv = 0;
+    sc->alpha = sc->omega = sc->beta = 0.;
+    sc->done = (sc->resid <= sc->tol || 0 >= max_iter);
+    sc->converged = sc->resid <= sc->tol;
+  }
+}
+
+__global__ void k_f1(int nblk, const double *part, KrylovScalars *sc)
+{
+  if(sc->done) return;
+  double o[1];
+  sum_partials(part, nblk, 1, o);
+  if(threadIdx.x == 0) sc->alpha = sc->rho / o[0];
+}
+
+__global__ void k_f2(int nblk, const double *part, KrylovScalars *sc)
+{
+  if(sc->done) return;
+  double o[3];
+  sum_partials(part, nblk, 3, o);
+  if(threadIdx.x == 0) {
+    double rs = sqrt(o[2]);
+    if(rs <= sc->tol) {   /* monitor.finished(s) */
+      sc->s_conv = 1;
+      sc->resid = rs;
+      sc->converged = 1;
+    }
+    sc->omega = o[0] / o[1];
+  }
+}
+
+__global__ void k_f3(int nblk, const double *part, KrylovScalars *sc)
+{
+  if(sc->done) return;
+  double o[2];
+  sum_partials(part, nblk, 2, o);
+  if(threadIdx.x == 0) {
+    if(sc->s_conv) { sc->done = 1; return; }
+    double rho_new = o[0];
+    sc->beta = (rho_new / sc->rho) * (sc->alpha / sc->omega);
+    sc->rho = rho_new;
+    sc->iter += 1;
+    sc->resid = sqrt(o[1]);
+    sc->converged = sc->resid <= sc->tol;
+    sc->done = sc->converged || sc->iter >= sc->max_iter;
+  }
+}
+
+/* ---------------------------------------------------------------------
+ * Sweeps, templated on the vector type (double for the plain form, float
+ * for the inner solve of the mixed form; dots accumulate in double).
+ * ------------------------------------------------------------------- */
+template<typename T>
+__device__ inline double mval(const T *__restrict__ y,
+  const uint8_t *__restrict__ fl, const double *dinv, int idx)
+{
+  return dinv[fl[idx] & 127] * (double) y[idx];
+}
+
+/* r = b/nrm - A (x/nrm); x and b are scaled in place; r* = p = r; v = 0 */
+template<typename T>
+__global__ void __launch_bounds__(256)
+k_init(PPGeom G, DinvTable tab, const KrylovScalars *sc,
+  T *__restrict__ b, T *__restrict__ x, const uint8_t *__restrict__ fl,
+  T *__restrict__ r, T *__restrict__ rs, T *__restrict__ p,
+  T *__restrict__ v, double *__restrict__ part, int scale_inputs)
+{
+  __shared__ double sh[4];
+  __shared__ double dinv[256];
+  const int tid = threadIdx.x + threadIdx.y * PX;
+  dinv[tid] = tid < 128 ? tab.d[tid] : tab.g[tid - 128];
+  __syncthreads();
+  const int lb = xcd_block(G.nblk);
+  TileIdx Ti = tile_of(G, lb);
+  const double inv = scale_inputs ? 1. / sc->bnorm : 1.;
+  double bb = 0., rr = 0.;
+  if(Ti.i < G.nx && Ti.j < G.ny) {
+    for(int k = Ti.k0; k < Ti.k1; k++) {
+      const int C = Ti.i + Ti.j*G.nx + k*G.nx*G.ny;
+      Nbr o = neighbours(G, Ti.i, Ti.j, k);
+      uint8_t f = fl[C];
+      double xc = (double) x[C] * inv;
+      double Ax = apply_row(f, G, xc,
+        (double) x[C+o.w] * inv, (double) x[C+o.e] * inv,
+        (double) x[C+o.s] * inv, (double) x[C+o.n] * inv,
+        (double) x[C+o.b] * inv, (double) x[C+o.t] * inv,
+        diag_of(dinv, f));
+      double bc_ = (double) b[C] * inv;
+      T rv = (T) (1. * bc_ + -1. * Ax);
+      r[C] = rv; rs[C] = rv; p[C] = rv; v[C] = (T) 0;
+      bb += bc_ * bc_;
+      rr += (double) rv * (double) rv;
+    }
+  }
+  double t0 = bb_block_sum(bb, sh);
+  double t1 = bb_block_sum(rr, sh);
+  if(tid == 0) { part[2*lb] = t0; part[2*lb+1] = t1; }
+}
+
+/* in-place scaling of x and b by 1/nrm (after k_init has used them) */
+template<typename T>
+__global__ void __launch_bounds__(256)
+k_scale2(size_t n, const KrylovScalars *sc, T *__restrict__ x,
+  T *__restrict__ b)
+{
+  const double inv = 1. / sc->bnorm;
+  size_t stride = (size_t) gridDim.x * blockDim.x;
+  for(size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+    x[q] = (T) ((double) x[q] * inv);
+    b[q] = (T) ((double) b[q] * inv);
+  }
+}
+
+/* K1: v = A M p, partial <r*, v> */
+template<typename T>
+__global__ void __launch_bounds__(256)
+k_spmv_p(PPGeom G, DinvTable tab, const KrylovScalars *sc,
+  const T *__restrict__ p, const T *__restrict__ rs,
+  const uint8_t *__restrict__ fl, T *__restrict__ v,
+  double *__restrict__ part)
+{
+  if(sc->done) return;
+  __shared__ double sh[4];
+  __shared__ double dinv[256];
+  const int tid = threadIdx.x + threadIdx.y * PX;
+  dinv[tid] = tid < 128 ? tab.d[tid] : tab.g[tid - 128];
+  __syncthreads();
+  const int lb = xcd_block(G.nblk);
+  TileIdx Ti = tile_of(G, lb);
+  double acc = 0.;
+  if(Ti.i < G.nx && Ti.j < G.ny) {
+    for(int k = Ti.k0; k < Ti.k1; k++) {
+      const int C = Ti.i + Ti.j*G.nx + k*G.nx*G.ny;
+      Nbr o = neighbours(G, Ti.i, Ti.j, k);
+      uint8_t f = fl[C];
+      double vv = apply_row(f, G, mval(p, fl, dinv, C),
+        mval(p, fl, dinv, C+o.w), mval(p, fl, dinv, C+o.e),
+        mval(p, fl, dinv, C+o.s), mval(p, fl, dinv, C+o.n),
+        mval(p, fl, dinv, C+o.b), mval(p, fl, dinv, C+o.t),
+        diag_of(dinv, f));
+      T vt = (T) vv;
+      v[C] = vt;
+      acc += (double) rs[C] * (double) vt;
+    }
+  }
+  double t = bb_block_sum(acc, sh);
+  if(tid == 0) part[lb] = t;
+}
+
+/* K2: s = r - alpha v (recomputed at the neighbours), t = A M s,
+ * partials <t,s>, <t,t>, <s,s> */
+template<typename T>
+__global__ void __launch_bounds__(256)
+k_spmv_s(PPGeom G, DinvTable tab, const KrylovScalars *sc,
+  const T *__restrict__ r, const T *__restrict__ v,
+  const uint8_t *__restrict__ fl, T *__restrict__ t,
+  double *__restrict__ part)
+{
+  if(sc->done) return;
+  __shared__ double sh[4];
+  __shared__ double dinv[256];
+  const int tid = threadIdx.x + threadIdx.y * PX;
+  dinv[tid] = tid < 128 ? tab.d[tid] : tab.g[tid - 128];
+  __syncthreads();
+  const int lb = xcd_block(G.nblk);
+  TileIdx Ti = tile_of(G, lb);
+  const double alpha = sc->alpha;
+  double ts = 0., tt = 0., ss = 0.;
+  if(Ti.i < G.nx && Ti.j < G.ny) {
+    for(int k = Ti.k0; k < Ti.k1; k++) {
+      const int C = Ti.i + Ti.j*G.nx + k*G.nx*G.ny;
+      Nbr o = neighbours(G, Ti.i, Ti.j, k);
+      uint8_t f = fl[C];
+#define SVAL(I) ((T) (1. * (double) r[I] + -alpha * (double) v[I]))
+#define MS(I) (dinv[fl[I] & 127] * (double) SVAL(I))
+      double sc_ = (double) SVAL(C);
+      double tv = apply_row(f, G, dinv[f & 127] * sc_,
+        MS(C+o.w), MS(C+o.e), MS(C+o.s), MS(C+o.n), MS(C+o.b), MS(C+o.t),
+        diag_of(dinv, f));
+#undef SVAL
+#undef MS
+      T tt_ = (T) tv;
+      t[C] = tt_;
+      ts += (double) tt_ * sc_;
+      tt += (double) tt_ * (double) tt_;
+      ss += sc_ * sc_;
+    }
+  }
+  double a = bb_block_sum(ts, sh);
+  double b = bb_block_sum(tt, sh);
+  double c = bb_block_sum(ss, sh);
+  if(tid == 0) { part[3*lb] = a; part[3*lb+1] = b; part[3*lb+2] = c; }
+}
+
+/* K3: x += alpha M p + omega M s; r = s - omega t; partials <r*,r>, <r,r>.
+ * After an s-convergence only x += alpha M p is applied. */
+template<typename T>
+__global__ void __launch_bounds__(256)
+k_update_xr(PPGeom G, DinvTable tab, const KrylovScalars *sc,
+  T *__restrict__ x, const T *__restrict__ p, T *__restrict__ r,
+  const T *__restrict__ v, const T *__restrict__ t, const T *__restrict__ rs,
+  const uint8_t *__restrict__ fl, double *__restrict__ part)
+{
+  if(sc->done) return;
+  __shared__ double sh[4];
+  __shared__ double dinv[256];
+  const int tid = threadIdx.x + threadIdx.y * PX;
+  dinv[tid] = tid < 128 ? tab.d[tid] : tab.g[tid - 128];
+  __syncthreads();
+  const int lb = xcd_block(G.nblk);
+  TileIdx Ti = tile_of(G, lb);
+  const double alpha = sc->alpha, omega = sc->omega;
+  const int sconv = sc->s_conv;
+  double a0 = 0., a1 = 0.;
+  if(Ti.i < G.nx && Ti.j < G.ny) {
+    for(int k = Ti.k0; k < Ti.k1; k++) {
+      const int C = Ti.i + Ti.j*G.nx + k*G.nx*G.ny;
+      double di = dinv[fl[C] & 127];
+      double Mp = di * (double) p[C];
+      if(sconv) {
+        x[C] = (T) (1. * (double) x[C] + alpha * Mp);
+        continue;
+      }
+      T s = (T) (1. * (double) r[C] + -alpha * (double) v[C]);
+      double Ms = di * (double) s;
+      x[C] = (T) (1. * (double) x[C] + alpha * Mp + omega * Ms);
+      T rn = (T) (1. * (double) s + -omega * (double) t[C]);
+      r[C] = rn;
+      a0 += (double) rs[C] * (double) rn;
+      a1 += (double) rn * (double) rn;
+    }
+  }
+  double b0 = bb_block_sum(a0, sh);
+  double b1 = bb_block_sum(a1, sh);
+  if(tid == 0) { part[2*lb] = b0; part[2*lb+1] = b1; }
+}
+
+/* K4: p = r + beta p - beta omega v */
+template<typename T>
+__global__ void __launch_bounds__(256)
+k_update_p_vec(size_t n, const KrylovScalars *sc, T *__restrict__ p,
+  const T *__restrict__ r, const T *__restrict__ v)
+{
+  if(sc->done) return;
+  const double beta = sc->beta, bo = -sc->beta * sc->omega;
+  size_t stride = (size_t) gridDim.x * blockDim.x;
+  for(size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+    p[q] = (T) (1. * (double) r[q] + beta * (double) p[q] + bo * (double) v[q]);
+}
+
+/* ---------------------------------------------------------------------
+ * Mean removal and copy back: x *= nrm; x -= mean(x); phi(interior) = x
+ * ------------------------------------------------------------------- */
+__global__ void __launch_bounds__(256)
+k_unscale_sum(size_t n, const KrylovScalars *sc, double *__restrict__ x,
+  double *__restrict__ part)
+{
+  __shared__ double sh[4];
+  const double nrm = sc->bnorm;
+  double s = 0.;
+  size_t stride = (size_t) gridDim.x * blockDim.x;
+  for(size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+    double val = x[q] * nrm;
+    x[q] = val;
+    s += val;
+  }
+  double t = bb_block_sum(s, sh);
+  if(threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void k_f_mean(int nblk, const double *part, double *out, double n)
+{
+  double o[1];
+  sum_partials(part, nblk, 1, o);
+  if(threadIdx.x == 0) out[0] = o[0] / n;
+}
+
+__global__ void __launch_bounds__(256)
+k_copy_back(PPGeom G, BBGeom B, const double *__restrict__ mean,
+  const double *__restrict__ x, double *__restrict__ phi)
+{
+  const int lb = blockIdx.x;
+  TileIdx Ti = tile_of(G, lb);
+  if(Ti.i >= G.nx || Ti.j >= G.ny) return;
+  const double pa = mean[0];
+  for(int k = Ti.k0; k < Ti.k1; k++) {
+    const int C = Ti.i + Ti.j*G.nx + k*G.nx*G.ny;
+    phi[(Ti.i + 1) + (Ti.j + 1)*B.cs1 + (k + 1)*B.cs2] = -pa * 1. + x[C];
+  }
+}
+
+/* ---------------------------------------------------------------------
+ * Mixed-precision helpers.
+ *   FP64 residual: r = b - A x  (x, b FP64), writes r/|r| into FP32 and
+ *   partial |r|^2; FP64 correction x += scale * e (e FP32).
+ * ------------------------------------------------------------------- */
+__global__ void __launch_bounds__(256)
+k_resid64(PPGeom G, DinvTable tab, const double *__restrict__ b,
+  const double *__restrict__ x, const uint8_t *__restrict__ fl,
+  double *__restrict__ r, double *__restrict__ part)
+{
+  __shared__ double sh[4];
+  __shared__ double dinv[256];
+  const int tid = threadIdx.x + threadIdx.y * PX;
+  dinv[tid] = tid < 128 ? tab.d[tid] : tab.g[tid - 128];
+  __syncthreads();
+  const int lb = xcd_block(G.nblk);
+  TileIdx Ti = tile_of(G, lb);
+  double rr = 0.;
+  if(Ti.i < G.nx && Ti.j < G.ny) {
+    for(int k = Ti.k0; k < Ti.k1; k++) {
+      const int C = Ti.i + Ti.j*G.nx + k*G.nx*G.ny;
+      Nbr o = neighbours(G, Ti.i, Ti.j, k);
+      uint8_t f = fl[C];
+      double Ax = apply_row(f, G, x[C], x[C+o.w], x[C+o.e], x[C+o.s],
+        x[C+o.n], x[C+o.b], x[C+o.t], diag_of(dinv, f));
+      double rv = b[C] - Ax;
+      r[C] = rv;
+      rr += rv * rv;
+    }
+  }
+  double t = bb_block_sum(rr, sh);
+  if(tid == 0) part[lb] = t;
+}
+
+/* FP32 copies for the inner solve: rhs32 = r / |r|, x32 = 0 */
+__global__ void __launch_bounds__(256)
+k_to32(size_t n, const double *__restrict__ scal, const double *__restrict__ r,
+  float *__restrict__ rf, float *__restrict__ xf)
+{
+  const double inv = 1. / scal[0];
+  size_t stride = (size_t) gridDim.x * blockDim.x;
+  for(size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+    rf[q] = (float) (r[q] * inv);
+    xf[q] = 0.f;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_correct64(size_t n, const double *__restrict__ scal,
+  const float *__restrict__ e, double *__restrict__ x)
+{
+  const double s = scal[0];
+  size_t stride = (size_t) gridDim.x * blockDim.x;
+  for(size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+    x[q] += s * (double) e[q];
+}
+
+__global__ void k_f_rnorm(int nblk, const double *part, double *out)
+{
+  double o[1];
+  sum_partials(part, nblk, 1, o);
+  if(threadIdx.x == 0) out[0] = sqrt(o[0]);
+}
+
+/* ---------------------------------------------------------------------
+ * Host drivers
+ * ------------------------------------------------------------------- */
+static int elementwise_blocks(size_t n)
+{
+  size_t b = (n + 255) / 256;
+  if(b > 2048) b = 2048;
+  return (int) (b ? b : 1);
+}
+
+/* One BiCGSTAB solve of A x = b on vectors of type T, with x already
+ * holding the initial guess and b the right-hand side (both already
+ * normalised when `scale_inputs` is 0).  Returns the iteration count. */
+template<typename T>
+static void bicgstab_run(const PPGeom &G, const DinvTable &tab, T *b, T *x,
+  T *r, T *rs, T *p, T *v, T *t, double rel_tol, int max_iter,
+  int scale_inputs, int poll, int kinds[4])
+{
+  BBWorkspace *ws = &g_bb.ws;
+  hipStream_t st = g_bb.stream;
+  dim3 blk(PX, PY);
+  size_t n = ws->n;
+  int eb = elementwise_blocks(n);
+  k_init<T><<<G.nblk, blk, 0, st>>>(G, tab, ws->sc, b, x, ws->flags, r, rs,
+    p, v, ws->partials, scale_inputs);
+  BB_CHECK_LAUNCH();
+  if(scale_inputs) {
+    k_scale2<T><<<eb, 256, 0, st>>>(n, ws->sc, x, b);
+    BB_CHECK_LAUNCH();
+  }
+  k_f_init<<<1, 256, 0, st>>>(G.nblk, ws->partials, ws->sc, rel_tol, max_iter);
+  BB_CHECK_LAUNCH();
+  for(int it = 0; it < max_iter + 1; it++) {
+    bb_prof_begin(kinds[0]);
+    k_spmv_p<T><<<G.nblk, blk, 0, st>>>(G, tab, ws->sc, p, rs, ws->flags, v,
+      ws->partials);
+    BB_CHECK_LAUNCH();
+    bb_prof_end(kinds[0]);
+    k_f1<<<1, 256, 0, st>>>(G.nblk, ws->partials, ws->sc);
+    BB_CHECK_LAUNCH();
+    bb_prof_begin(kinds[1]);
+    k_spmv_s<T><<<G.nblk, blk, 0, st>>>(G, tab, ws->sc, r, v, ws->flags, t,
+      ws->partials);
+    BB_CHECK_LAUNCH();
+    bb_prof_end(kinds[1]);
+    k_f2<<<1, 256, 0, st>>>(G.nblk, ws->partials, ws->sc);
+    BB_CHECK_LAUNCH();
+    bb_prof_begin(kinds[2]);
+    k_update_xr<T><<<G.nblk, blk, 0, st>>>(G, tab, ws->sc, x, p, r, v, t, rs,
+      ws->flags, ws->partials);
+    BB_CHECK_LAUNCH();
+    bb_prof_end(kinds[2]);
+    k_f3<<<1, 256, 0, st>>>(G.nblk, ws->partials, ws->sc);
+    BB_CHECK_LAUNCH();
+    bb_prof_begin(kinds[3]);
+    k_update_p_vec<T><<<eb, 256, 0, st>>>(n, ws->sc, p, r, v);
+    BB_CHECK_LAUNCH();
+    bb_prof_end(kinds[3]);
+    if((it + 1) % poll == 0 || it == max_iter) {
+      BB_CHECK(hipMemcpyAsync(ws->sc_host, ws->sc, sizeof(KrylovScalars),
+        hipMemcpyDeviceToHost, st));
+      BB_CHECK(hipStreamSynchronize(st));
+      if(ws->sc_host->done) break;
+    }
+  }
+  BB_CHECK(hipMemcpyAsync(ws->sc_host, ws->sc, sizeof(KrylovScalars),
+    hipMemcpyDeviceToHost, st));
+  BB_CHECK(hipStreamSynchronize(st));
+}
+
+static int g_poll = 4;
+
+extern "C" void bb_set_poll_interval(int n) { g_poll = n > 0 ? n : 1; }
+
+static void solve_fp64(const PPGeom &G, const DinvTable &tab)
+{
+  BBWorkspace *ws = &g_bb.ws;
+  int kinds[4] = {BB_K_SPMV_P, BB_K_SPMV_S, BB_K_UPDATE_XR, BB_K_UPDATE_P};
+  bicgstab_run<double>(G, tab, _rhs_p[0], ws->x, ws->r, ws->rs, ws->p, ws->v,
+    ws->t, pp_residual, pp_max_iter, 1, g_poll, kinds);
+  KrylovScalars *h = ws->sc_host;
+  g_bb.stats.iterations = h->iter;
+  g_bb.stats.outer_steps = 1;
+  g_bb.stats.residual = h->resid;
+  g_bb.stats.converged = h->converged;
+}
+
+/* FP64 refinement around FP32 BiCGSTAB.  Inner solves run to a relative
+ * reduction of `inner_tol` (or `inner_max` iterations); each outer step
+ * recomputes the true residual in FP64. */
+static double g_inner_tol = 1e-4;
+static int g_inner_max = 2000;
+static int g_outer_max = 50;
+
+extern "C" void bb_set_mixed_params(double inner_tol, int inner_max, int outer_max)
+{
+  if(inner_tol > 0) g_inner_tol = inner_tol;
+  if(inner_max > 0) g_inner_max = inner_max;
+  if(outer_max > 0) g_outer_max = outer_max;
+}
+
+static void solve_mixed(const PPGeom &G, const DinvTable &tab)
+{
+  BBWorkspace *ws = &g_bb.ws;
+  hipStream_t st = g_bb.stream;
+  size_t n = ws->n;
+  int eb = elementwise_blocks(n);
+  dim3 blk(PX, PY);
+  double *b = _rhs_p[0];
+  /* normalise b and x like the plain form (b := rhs/|rhs|) */
+  k_scale2<double><<<eb, 256, 0, st>>>(n, ws->sc, ws->x, b);
+  BB_CHECK_LAUNCH();
+  /* |b| (close to 1) for the tolerance */
+  int kinds[4] = {BB_K_SPMV_P32, BB_K_SPMV_S32, BB_K_UPDATE_XR32,
+    BB_K_UPDATE_P32};
+  double *rn = ws->red + 24;   /* device: |r| of the current outer step */
+  double bnorm = 0.;
+  int total_inner = 0, outer = 0, converged = 0;
+  double resid = 0.;
+  for(outer = 0; outer <= g_outer_max; outer++) {
+    bb_prof_begin(BB_K_RESID64);
+    k_resid64<<<G.nblk, blk, 0, st>>>(G, tab, b, ws->x, ws->flags, ws->r,
+      ws->partials);
+    BB_CHECK_LAUNCH();
+    bb_prof_end(BB_K_RESID64);
+    k_f_rnorm<<<1, 256, 0, st>>>(G.nblk, ws->partials, rn);
+    BB_CHECK_LAUNCH();
+    if(outer == 0) {
+      /* |b|: reuse the residual kernel on x = 0 would cost a pass; the
+       * normalised rhs has |b| = 1 up to rounding, computed exactly here */
+      k_resid64<<<G.nblk, blk, 0, st>>>(G, tab, b, ws->t /* zero */, ws->flags,
+        ws->v, ws->partials);
+      BB_CHECK_LAUNCH();
+      k_f_rnorm<<<1, 256, 0, st>>>(G.nblk, ws->partials, rn + 1);
+      BB_CHECK_LAUNCH();
+    }
+    BB_CHECK(hipMemcpyAsync(ws->red_host, rn, 2 * sizeof(double),
+      hipMemcpyDeviceToHost, st));
+    BB_CHECK(hipStreamSynchronize(st));
+    if(outer == 0) bnorm = ws->red_host[1];
+    resid = ws->red_host[0];
+    double tol = pp_residual * bnorm;
+    if(resid <= tol) { converged = 1; break; }
+    if(outer == g_outer_max || total_inner >= pp_max_iter) break;
+    /* inner: A e = r/|r| in FP32 from e = 0 */
+    k_to32<<<eb, 256, 0, st>>>(n, rn, ws->r, ws->bf, ws->xf);
+    BB_CHECK_LAUNCH();
+    double inner_tol = g_inner_tol;
+    /* do not solve far below what the outer test needs */
+    double need = 0.5 * tol / resid;
+    if(need > inner_tol) inner_tol = need;
+    int imax = g_inner_max;
+    if(imax > pp_max_iter - total_inner) imax = pp_max_iter - total_inner;
+    bicgstab_run<float>(G, tab, ws->bf, ws->xf, ws->rf, ws->rsf, ws->pf, ws->vf,
+      ws->tf, inner_tol, imax, 0, g_poll, kinds);
+    total_inner += ws->sc_host->iter + (ws->sc_host->s_conv ? 1 : 0);
+    bb_prof_begin(BB_K_CORRECT64);
+    k_correct64<<<eb, 256, 0, st>>>(n, rn, ws->xf, ws->x);
+    BB_CHECK_LAUNCH();
+    bb_prof_end(BB_K_CORRECT64);
+  }
+  g_bb.stats.iterations = total_inner;
+  g_bb.stats.outer_steps = outer;
+  g_bb.stats.residual = bnorm > 0 ? resid / bnorm : resid;
+  g_bb.stats.converged = converged;
+  ws->sc_host->iter = total_inner;
+  ws->sc_host->resid = g_bb.stats.residual;
+  ws->sc_host->converged = converged;
+}
+
+extern "C" void cuda_PP_bicgstab(int rank)
+{
+  bb_ensure_workspace();
+  BBWorkspace *ws = &g_bb.ws;
+  hipStream_t st = g_bb.stream;
+  PPGeom G = pp_geom();
+  BBGeom B = bb_geom();
+  DinvTable tab = make_dinv(G.ddx, G.ddy, G.ddz);
+  dim3 blk(PX, PY);
+
+  bb_prof_begin(BB_K_RHS);
+  k_pp_setup<<<G.nblk, blk, 0, st>>>(G, B, rho_f / dt, nparts > 0,
+    _u_star[0], _v_star[0], _w_star[0], _phi[0], _phase[0], _phase_shell[0],
+    _flag_u[0], _flag_v[0], _flag_w[0], _rhs_p[0], ws->x, ws->flags,
+    ws->partials);
+  BB_CHECK_LAUNCH();
+  bb_prof_end(BB_K_RHS);
+  k_f_norm<<<1, 256, 0, st>>>(G.nblk, ws->partials, ws->sc);
+  BB_CHECK_LAUNCH();
+
+  if(g_bb.solver_mode == BB_SOLVER_MIXED) {
+    /* a zero FP64 vector for the |b| pass of the refinement loop */
+    BB_CHECK(hipMemsetAsync(ws->t, 0, ws->n * sizeof(double), st));
+    solve_mixed(G, tab);
+  } else {
+    solve_fp64(G, tab);
+  }
+
+  char nam[FILE_NAME_SIZE] = "solver_expd.rec";
+  char namp[FILE_NAME_SIZE] = "solver_prec.rec";
+  recorder_bicgstab(rank == 0 ? nam : namp, g_bb.stats.iterations,
+    g_bb.stats.residual);
+  if(!g_bb.stats.converged) {
+    printf("The pressure-Poisson equation did not converge.              \n");
+    fflush(stdout);
+    if(g_bb.exit_on_fail) exit(EXIT_FAILURE);
+  }
+
+  /* unnormalise, subtract the mean over all cells, copy into phi */
+  int eb = elementwise_blocks(ws->n);
+  k_unscale_sum<<<eb, 256, 0, st>>>(ws->n, ws->sc, ws->x, ws->partials);
+  BB_CHECK_LAUNCH();
+  k_f_mean<<<1, 256, 0, st>>>(eb, ws->partials, ws->red + 32, (double) ws->n);
+  BB_CHECK_LAUNCH();
+  k_copy_back<<<G.nblk, blk, 0, st>>>(G, B, ws->red + 32, ws->x, _phi[0]);
+  BB_CHECK_LAUNCH();
+}

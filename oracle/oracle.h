@@ -1,0 +1,160 @@
+/*
+ * oracle.h -- serial FP64 C restatement of the reference Bluebottle
+ * per-timestep path.  TEST INFRASTRUCTURE ONLY: it is the checker that the
+ * HIP library is compared against.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's CPU-baseline leg may load it; the product path never does.
+ *
+ * Each function cites the reference kernel/host function it restates.  The
+ * oracle keeps its own copy of the simulation state (orc_state), laid out
+ * exactly like the reference's host globals (ghosted, x-fastest arrays), so
+ * that a test can copy the library's state in, run one entry point on both
+ * sides, and compare array by array.
+ *
+ * Parity status per function (pins are listed in DESIGN.md and exercised in
+ * tests/test_oracle_*.py): see the comment above each prototype.
+ */
+#ifndef BB_ORACLE_H
+#define BB_ORACLE_H
+
+#include <stddef.h>
+#include "../include/bluebottle.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_state {
+  dom_struct dom;             /* the single subdomain (== Dom) */
+  BC bc;
+  g_struct g;
+  gradP_struct gradP;
+  real rho_f, mu, nu;
+  real dt, dt0, ttime, CFL;
+  real pp_residual, lamb_residual, lamb_relax, lamb_cut;
+  int pp_max_iter, lamb_max_iter, out_plane, stepnum, init_cond;
+  real p_tDelay, g_tDelay, vel_tDelay;
+  real Kp, Ki, Kd, pid_int, pid_back;
+  /* Gcc arrays */
+  real *p0, *p, *phi;
+  int *phase, *phase_shell;
+  /* Gfx/Gfy/Gfz arrays */
+  real *u, *u0, *v, *v0, *w, *w0;
+  real *f_x, *f_y, *f_z;
+  real *diff0_u, *diff0_v, *diff0_w, *conv0_u, *conv0_v, *conv0_w;
+  real *diff_u, *diff_v, *diff_w, *conv_u, *conv_v, *conv_w;
+  real *u_star, *v_star, *w_star;
+  int *flag_u, *flag_v, *flag_w;
+  /* solver vector (no ghosts, Gcc.s3) */
+  real *rhs_p;
+  /* particles */
+  int nparts;
+  real interactionLengthRatio;
+  part_struct *parts;
+  dom_struct binDom;
+  int coeff_stride;
+  real *coeff[18];            /* pnm_re, pnm_im, phinm_re, phinm_im,
+                                 chinm_re, chinm_im, then the same six
+                                 for *0 and for *00 */
+  /* telemetry of the last Poisson solve */
+  int last_iters;
+  real last_resid;
+  int last_converged;
+} orc_state;
+
+/* Names of coeff[] entries, in order. */
+extern const char *orc_coeff_names[18];
+
+/* --- state management (oracle_state.c) --- */
+orc_state *orc_create(const dom_struct *Dom, int nparts, int coeff_stride);
+void orc_free(orc_state *s);
+/* Look up an array by its reference global name; returns its element count
+ * in *n and its element type in *is_int (0 = real, 1 = int). */
+void *orc_array(orc_state *s, const char *name, size_t *n, int *is_int);
+size_t orc_sizeof_state(void);
+
+/* --- field path (oracle_field.c) --- */
+/* src/bluebottle_kernel.cu:1504-2003 via src/cuda_bluebottle.cu:2930. */
+void orc_U_star_2(orc_state *s);
+/* src/cuda_bluebottle.cu:1549-3420 with the BC_* kernels at
+ * src/bluebottle_kernel.cu:26-1248 (PRECURSOR faces unsupported). */
+void orc_dom_BC(orc_state *s);
+void orc_dom_BC_star(orc_state *s);
+void orc_dom_BC_phi(orc_state *s);
+void orc_dom_BC_p(orc_state *s);
+/* src/cuda_bluebottle.cu:4122-4314, src/bluebottle_kernel.cu:2133-2245. */
+void orc_solvability(orc_state *s);
+/* src/cuda_bluebottle.cu:3422-3497, src/bluebottle_kernel.cu:1250-1302. */
+void orc_project(orc_state *s);
+/* src/cuda_bluebottle.cu:3616-3671, src/bluebottle_kernel.cu:1326-1338. */
+void orc_update_p(orc_state *s);
+/* src/cuda_bluebottle.cu:3500-3577. */
+real orc_find_dt(orc_state *s);
+/* src/cuda_bluebottle.cu:3580-3613. */
+void orc_store_u(orc_state *s);
+/* src/cuda_bluebottle.cu:3674-3814 (PID branch included). */
+void orc_compute_forcing(orc_state *s);
+
+/* --- Poisson solve (oracle_poisson.c) --- */
+/* src/bicgstab_kernel.cu:483-562 */
+void orc_PP_rhs(orc_state *s);
+/* src/particle_kernel.cu:840-940 (non-STEPS branch). */
+void orc_part_BC_p(orc_state *s);
+/* y = A x with A the 13-diagonal matrix of src/bicgstab_kernel.cu:1654-1856
+ * and the offsets of src/cuda_bicgstab.cu:735-747, applied as CUSP's DIA
+ * SpMV would (x, y: no-ghost vectors of length Gcc.s3). */
+void orc_pp_matvec(const orc_state *s, const real *x, real *y);
+/* The 13 DIA coefficient rows for cell C (values[C + d*pitch], d=0..12). */
+void orc_pp_coeffs(const orc_state *s, int C, real vals[13]);
+/* Jacobi-preconditioned BiCGSTAB with CUSP's monitor semantics
+ * (src/cuda_bicgstab.cu:1016-1018; CUSP itself is not vendored, so this is
+ * the published right-preconditioned algorithm; iterate sequence parity
+ * unpinned, converged-solution parity pinned by the residual). */
+int orc_bicgstab(const orc_state *s, const real *b, real *x, int max_iter,
+  real rel_tol, real *resid_out, int *converged);
+/* The whole entry point: src/cuda_bicgstab.cu:715-1073. */
+void orc_PP_bicgstab(orc_state *s);
+
+/* --- particles (oracle_particle.c) --- */
+/* src/cuda_particle.cu:357-896 + src/particle_kernel.cu:25-533. */
+void orc_build_cages(orc_state *s);
+/* src/cuda_particle.cu:899-992 + src/particle_kernel.cu:535-838. */
+void orc_part_BC(orc_state *s);
+void orc_part_BC_star(orc_state *s);
+/* src/cuda_particle.cu:1019-1042 + src/particle_kernel.cu:942-1018. */
+void orc_part_p_fill(orc_state *s);
+/* src/cuda_bluebottle.cu:4822-4900 + src/bluebottle_kernel.cu:3916-4022. */
+void orc_parts_internal(orc_state *s);
+/* src/cuda_particle.cu:1046-1081. */
+void orc_store_coeffs(orc_state *s);
+/* src/cuda_bluebottle.cu:4317-4554 + src/bluebottle_kernel.cu:2247-3617. */
+void orc_move_parts_sub(orc_state *s);
+void orc_move_parts(orc_state *s);
+/* Lamb analytic velocity at (r, theta, phi) about particle P
+ * (src/particle_kernel.cu:1311-1369). */
+void orc_lamb_vel(const orc_state *s, int order, real a, real r, real theta,
+  real phi, int P, real *Ux, real *Uy, real *Uz);
+
+/* --- quadrature (oracle_quad.c) --- */
+/* src/cuda_quadrature.cu:64-313 + src/quadrature_kernel.cu:25-576. */
+void orc_Lamb(orc_state *s);
+/* src/cuda_quadrature.cu:316-398 + src/quadrature_kernel.cu:578-654 with
+ * the cross-particle race removed (per-particle scratch). */
+real orc_lamb_err(orc_state *s);
+/* Node tables of the 26-point Lebedev rule (src/cuda_quadrature.cu:86-157). */
+void orc_lebedev_nodes(real theta[26], real phi[26], real weight[26]);
+/* Helpers exposed for closed-form tests. */
+real orc_Nnm(int n, int m);
+real orc_Pnm(int n, int m, real theta);
+void orc_xyz2rtp(real x, real y, real z, real *r, real *theta, real *phi);
+
+/* --- one full time step (oracle_step.c): src/bluebottle.c:598-711 --- */
+/* Returns the number of Lamb sub-iterations taken. */
+int orc_step(orc_state *s);
+/* The set-up sequence before the loop: src/bluebottle.c:445-545. */
+void orc_prepare(orc_state *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

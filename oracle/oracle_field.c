@@ -1,0 +1,524 @@
+/*
+ * oracle_field.c -- serial FP64 restatement of the reference's field
+ * kernels: predictor, ghost-cell BCs, solvability, projection, pressure
+ * update, time-step size, history shift, forcing.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Operation order inside each expression follows the reference kernels so
+ * that the oracle and a faithful device implementation agree to rounding.
+ */
+#include <math.h>
+#include <float.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+
+#define IDX(G, i, j, k) ((i) + (j) * (G).s1b + (k) * (G).s2b)
+
+/* ---------------------------------------------------------------------
+ * Predictor: src/bluebottle_kernel.cu:1504-2003 (u_star_2, v_star_2,
+ * w_star_2), launched for all three components by
+ * src/cuda_bluebottle.cu:2930-3026 with p0 as the pressure.
+ * ------------------------------------------------------------------- */
+void orc_U_star_2(orc_state *s)
+{
+  const dom_struct *d = &s->dom;
+  const grid_info Gc = d->Gcc, Gx = d->Gfx, Gy = d->Gfy, Gz = d->Gfz;
+  const real dt = s->dt, dt0 = s->dt0, nu = s->nu, rho_f = s->rho_f;
+  const real ab0 = 0.5 * dt / dt0;
+  const real ab = 1. + ab0;
+  const real ddx = 1. / d->dx, ddy = 1. / d->dy, ddz = 1. / d->dz;
+  const real *u0 = s->u0, *v0 = s->v0, *w0 = s->w0, *p = s->p0;
+  const int *phase = s->phase;
+
+  /* u: src/bluebottle_kernel.cu:1504-1669 */
+  for(int k = Gx.ks; k < Gx.ke; k++)
+  for(int j = Gx.js; j < Gx.je; j++)
+  for(int i = Gx.is; i < Gx.ie; i++) {
+    int C = IDX(Gx, i, j, k);
+    real us = (p[IDX(Gc, i-1, j, k)] - p[IDX(Gc, i, j, k)]) * ddx / rho_f;
+    real u011 = u0[IDX(Gx, i-1, j, k)], u111 = u0[C], u211 = u0[IDX(Gx, i+1, j, k)];
+    real u101 = u0[IDX(Gx, i, j-1, k)], u121 = u0[IDX(Gx, i, j+1, k)];
+    real v011 = v0[IDX(Gy, i-1, j, k)], v111 = v0[IDX(Gy, i, j, k)];
+    real v021 = v0[IDX(Gy, i-1, j+1, k)], v121 = v0[IDX(Gy, i, j+1, k)];
+    real u110 = u0[IDX(Gx, i, j, k-1)], u112 = u0[IDX(Gx, i, j, k+1)];
+    real w011 = w0[IDX(Gz, i-1, j, k)], w111 = w0[IDX(Gz, i, j, k)];
+    real w012 = w0[IDX(Gz, i-1, j, k+1)], w112 = w0[IDX(Gz, i, j, k+1)];
+
+    real duudx = (u211 + u111)*(u211 + u111) - (u111 + u011)*(u111 + u011);
+    duudx *= 0.25 * ddx;
+    real duvdy = (u121 + u111)*(v121 + v021) - (u111 + u101)*(v111 + v011);
+    duvdy *= 0.25 * ddy;
+    real duwdz = (u112 + u111)*(w112 + w012) - (u111 + u110)*(w111 + w011);
+    duwdz *= 0.25 * ddz;
+    real c = duudx + duvdy + duwdz;
+    if(dt0 > 0) us += (-ab * c + ab0 * s->conv0_u[C]);
+    else us += -c;
+
+    real dud1 = (u211 - u111) * ddx;
+    real dud0 = (u111 - u011) * ddx;
+    real ddudxx = (dud1 - dud0) * ddx;
+    dud1 = (u121 - u111) * ddy;
+    dud0 = (u111 - u101) * ddy;
+    real ddudyy = (dud1 - dud0) * ddy;
+    dud1 = (u112 - u111) * ddz;
+    dud0 = (u111 - u110) * ddz;
+    real ddudzz = (dud1 - dud0) * ddz;
+    real df = nu * (ddudxx + ddudyy + ddudzz);
+    if(dt0 > 0) us += (ab * df - ab0 * s->diff0_u[C]);
+    else us += df;
+    us += s->f_x[C];
+    us *= dt;
+    us += u111;
+    us *= (phase[IDX(Gc, i-1, j, k)] < 0 && phase[IDX(Gc, i, j, k)] < 0);
+    s->u_star[C] = us;
+    s->conv_u[C] = c;
+    s->diff_u[C] = df;
+  }
+
+  /* v: src/bluebottle_kernel.cu:1673-1836 */
+  for(int k = Gy.ks; k < Gy.ke; k++)
+  for(int j = Gy.js; j < Gy.je; j++)
+  for(int i = Gy.is; i < Gy.ie; i++) {
+    int C = IDX(Gy, i, j, k);
+    real vs = (p[IDX(Gc, i, j-1, k)] - p[IDX(Gc, i, j, k)]) * ddy / rho_f;
+    real v101 = v0[IDX(Gy, i, j-1, k)], v111 = v0[C], v121 = v0[IDX(Gy, i, j+1, k)];
+    real v110 = v0[IDX(Gy, i, j, k-1)], v112 = v0[IDX(Gy, i, j, k+1)];
+    real w101 = w0[IDX(Gz, i, j-1, k)], w111 = w0[IDX(Gz, i, j, k)];
+    real w102 = w0[IDX(Gz, i, j-1, k+1)], w112 = w0[IDX(Gz, i, j, k+1)];
+    real v011 = v0[IDX(Gy, i-1, j, k)], v211 = v0[IDX(Gy, i+1, j, k)];
+    real u101 = u0[IDX(Gx, i, j-1, k)], u111 = u0[IDX(Gx, i, j, k)];
+    real u201 = u0[IDX(Gx, i+1, j-1, k)], u211 = u0[IDX(Gx, i+1, j, k)];
+
+    real dvudx = (v211 + v111)*(u211 + u201) - (v111 + v011)*(u111 + u101);
+    dvudx *= 0.25 * ddx;
+    real dvvdy = (v121 + v111)*(v121 + v111) - (v111 + v101)*(v111 + v101);
+    dvvdy *= 0.25 * ddy;
+    real dvwdz = (v112 + v111)*(w112 + w102) - (v111 + v110)*(w111 + w101);
+    dvwdz *= 0.25 * ddz;
+    real c = dvudx + dvvdy + dvwdz;
+    if(dt0 > 0) vs += (-ab * c + ab0 * s->conv0_v[C]);
+    else vs += -c;
+
+    real dvd1 = (v211 - v111) * ddx;
+    real dvd0 = (v111 - v011) * ddx;
+    real ddvdxx = (dvd1 - dvd0) * ddx;
+    dvd1 = (v121 - v111) * ddy;
+    dvd0 = (v111 - v101) * ddy;
+    real ddvdyy = (dvd1 - dvd0) * ddy;
+    dvd1 = (v112 - v111) * ddz;
+    dvd0 = (v111 - v110) * ddz;
+    real ddvdzz = (dvd1 - dvd0) * ddz;
+    real df = nu * (ddvdxx + ddvdyy + ddvdzz);
+    if(dt0 > 0) vs += (ab * df - ab0 * s->diff0_v[C]);
+    else vs += df;
+    vs += s->f_y[C];
+    vs *= dt;
+    vs += v111;
+    vs *= (phase[IDX(Gc, i, j-1, k)] < 0 && phase[IDX(Gc, i, j, k)] < 0);
+    s->v_star[C] = vs;
+    s->conv_v[C] = c;
+    s->diff_v[C] = df;
+  }
+
+  /* w: src/bluebottle_kernel.cu:1840-2003 */
+  for(int k = Gz.ks; k < Gz.ke; k++)
+  for(int j = Gz.js; j < Gz.je; j++)
+  for(int i = Gz.is; i < Gz.ie; i++) {
+    int C = IDX(Gz, i, j, k);
+    real ws = (p[IDX(Gc, i, j, k-1)] - p[IDX(Gc, i, j, k)]) * ddz / rho_f;
+    real w110 = w0[IDX(Gz, i, j, k-1)], w111 = w0[C], w112 = w0[IDX(Gz, i, j, k+1)];
+    real w011 = w0[IDX(Gz, i-1, j, k)], w211 = w0[IDX(Gz, i+1, j, k)];
+    real u110 = u0[IDX(Gx, i, j, k-1)], u111 = u0[IDX(Gx, i, j, k)];
+    real u210 = u0[IDX(Gx, i+1, j, k-1)], u211 = u0[IDX(Gx, i+1, j, k)];
+    real w101 = w0[IDX(Gz, i, j-1, k)], w121 = w0[IDX(Gz, i, j+1, k)];
+    real v110 = v0[IDX(Gy, i, j, k-1)], v111 = v0[IDX(Gy, i, j, k)];
+    real v120 = v0[IDX(Gy, i, j+1, k-1)], v121 = v0[IDX(Gy, i, j+1, k)];
+
+    real dwudx = (w211 + w111)*(u211 + u210) - (w111 + w011)*(u111 + u110);
+    dwudx *= 0.25 * ddx;
+    real dwvdy = (w121 + w111)*(v121 + v120) - (w111 + w101)*(v111 + v110);
+    dwvdy *= 0.25 * ddy;
+    real dwwdz = (w112 + w111)*(w112 + w111) - (w111 + w110)*(w111 + w110);
+    dwwdz *= 0.25 * ddz;
+    real c = dwudx + dwvdy + dwwdz;
+    if(dt0 > 0) ws += (-ab * c + ab0 * s->conv0_w[C]);
+    else ws += -c;
+
+    real dwd1 = (w211 - w111) * ddx;
+    real dwd0 = (w111 - w011) * ddx;
+    real ddwdxx = (dwd1 - dwd0) * ddx;
+    dwd1 = (w121 - w111) * ddy;
+    dwd0 = (w111 - w101) * ddy;
+    real ddwdyy = (dwd1 - dwd0) * ddy;
+    dwd1 = (w112 - w111) * ddz;
+    dwd0 = (w111 - w110) * ddz;
+    real ddwdzz = (dwd1 - dwd0) * ddz;
+    real df = nu * (ddwdxx + ddwdyy + ddwdzz);
+    if(dt0 > 0) ws += (ab * df - ab0 * s->diff0_w[C]);
+    else ws += df;
+    ws += s->f_z[C];
+    ws *= dt;
+    ws += w111;
+    ws *= (phase[IDX(Gc, i, j, k-1)] < 0 && phase[IDX(Gc, i, j, k)] < 0);
+    s->w_star[C] = ws;
+    s->conv_w[C] = c;
+    s->diff_w[C] = df;
+  }
+}
+
+/* ---------------------------------------------------------------------
+ * Ghost-cell boundary conditions: the BC_{p,u,v,w}_{W,E,S,N,B,T}_{P,D,N}
+ * kernels of src/bluebottle_kernel.cu:26-1248.  One face of one field is
+ * applied over the full ghosted transverse extent; faces are applied in
+ * the order W, E, S, N, B, T (src/cuda_bluebottle.cu:1565-2298).
+ *   axis: 0 = x, 1 = y, 2 = z;  hi: 0 = W/S/B, 1 = E/N/T;
+ *   normal: 1 if the field is staggered along `axis` (u in x, v in y,
+ *   w in z), else 0 (cell-centred along the axis, also p).
+ * ------------------------------------------------------------------- */
+static void bc_face(real *f, grid_info G, int axis, int hi, int type,
+  int normal, real bcv)
+{
+  int s1 = 1, s2, s3;                 /* strides along axis, t1, t2 */
+  int n1, n2;                         /* transverse extents incl. ghosts */
+  int s, e, sb, eb;                   /* start/end along the axis */
+  int str;
+  if(axis == 0) {
+    str = 1; s2 = G.s1b; s3 = G.s2b; n1 = G.jnb; n2 = G.knb;
+    s = G.is; e = G.ie; sb = G.isb; eb = G.ieb;
+  } else if(axis == 1) {
+    str = G.s1b; s2 = 1; s3 = G.s2b; n1 = G.inb; n2 = G.knb;
+    s = G.js; e = G.je; sb = G.jsb; eb = G.jeb;
+  } else {
+    str = G.s2b; s2 = 1; s3 = G.s1b; n1 = G.inb; n2 = G.jnb;
+    s = G.ks; e = G.ke; sb = G.ksb; eb = G.keb;
+  }
+  (void) s1;
+  for(int t2 = 0; t2 < n2; t2++)
+  for(int t1 = 0; t1 < n1; t1++) {
+    real *F = f + (long) t1 * s2 + (long) t2 * s3;
+#define A(n) F[(long) (n) * str]
+    if(type == PERIODIC) {
+      if(normal) {
+        if(!hi) { A(sb) = A(e-2); A(s) = A(e-1); }
+        else    { A(eb-1) = A(s+1); A(e-1) = A(s); }
+      } else {
+        if(!hi) A(sb) = A(e-1);
+        else    A(eb-1) = A(s);
+      }
+    } else if(type == DIRICHLET) {
+      if(normal) {
+        if(!hi) { A(sb) = 2. * bcv - A(s+1); A(s) = bcv; }
+        else    { A(eb-1) = 2. * bcv - A(e-2); A(e-1) = bcv; }
+      } else {
+        if(!hi) A(sb) = 8./3. * bcv - 2. * A(s) + 1./3. * A(s+1);
+        else    A(eb-1) = 8./3. * bcv - 2. * A(e-1) + 1./3. * A(e-2);
+      }
+    } else if(type == NEUMANN) {
+      if(!hi) A(sb) = A(s);
+      else    A(eb-1) = A(e-1);
+    }
+#undef A
+  }
+}
+
+/* BC type/value table for one field, faces in W,E,S,N,B,T order. */
+static void bc_table(const BC *b, char field, int type[6], real val[6])
+{
+  switch(field) {
+    case 'p':
+      type[0] = b->pW; type[1] = b->pE; type[2] = b->pS; type[3] = b->pN;
+      type[4] = b->pB; type[5] = b->pT;
+      for(int f = 0; f < 6; f++) val[f] = 0.;
+      break;
+    case 'u':
+      type[0] = b->uW; type[1] = b->uE; type[2] = b->uS; type[3] = b->uN;
+      type[4] = b->uB; type[5] = b->uT;
+      val[0] = b->uWD; val[1] = b->uED; val[2] = b->uSD; val[3] = b->uND;
+      val[4] = b->uBD; val[5] = b->uTD;
+      break;
+    case 'v':
+      type[0] = b->vW; type[1] = b->vE; type[2] = b->vS; type[3] = b->vN;
+      type[4] = b->vB; type[5] = b->vT;
+      val[0] = b->vWD; val[1] = b->vED; val[2] = b->vSD; val[3] = b->vND;
+      val[4] = b->vBD; val[5] = b->vTD;
+      break;
+    default:
+      type[0] = b->wW; type[1] = b->wE; type[2] = b->wS; type[3] = b->wN;
+      type[4] = b->wB; type[5] = b->wT;
+      val[0] = b->wWD; val[1] = b->wED; val[2] = b->wSD; val[3] = b->wND;
+      val[4] = b->wBD; val[5] = b->wTD;
+      break;
+  }
+}
+
+static void bc_field(const orc_state *s, real *f, char field)
+{
+  const dom_struct *d = &s->dom;
+  grid_info G = field == 'p' ? d->Gcc : field == 'u' ? d->Gfx
+    : field == 'v' ? d->Gfy : d->Gfz;
+  int normal_axis = field == 'u' ? 0 : field == 'v' ? 1 : field == 'w' ? 2 : -1;
+  int type[6];
+  real val[6];
+  bc_table(&s->bc, field, type, val);
+  int ext[6] = {d->W, d->E, d->S, d->N, d->B, d->T};
+  for(int face = 0; face < 6; face++) {
+    if(ext[face] != -1) continue;
+    int axis = face / 2, hi = face % 2;
+    bc_face(f, G, axis, hi, type[face], axis == normal_axis, val[face]);
+  }
+}
+
+void orc_dom_BC(orc_state *s)
+{
+  bc_field(s, s->p, 'p');
+  bc_field(s, s->u, 'u');
+  bc_field(s, s->v, 'v');
+  bc_field(s, s->w, 'w');
+}
+
+void orc_dom_BC_star(orc_state *s)
+{
+  bc_field(s, s->u_star, 'u');
+  bc_field(s, s->v_star, 'v');
+  bc_field(s, s->w_star, 'w');
+}
+
+void orc_dom_BC_phi(orc_state *s) { bc_field(s, s->phi, 'p'); }
+void orc_dom_BC_p(orc_state *s) { bc_field(s, s->p, 'p'); }
+
+/* ---------------------------------------------------------------------
+ * Solvability: src/cuda_bluebottle.cu:4122-4314 with surf_int_*_copy and
+ * plane_eps_* from src/bluebottle_kernel.cu:2133-2245.  The reference sums
+ * with thrust::reduce (order unspecified); here the sum runs in array
+ * order (first the negated low face, then the high face).
+ * ------------------------------------------------------------------- */
+void orc_solvability(orc_state *s)
+{
+  const dom_struct *d = &s->dom;
+  const grid_info Gx = d->Gfx, Gy = d->Gfy, Gz = d->Gfz;
+  real eps_x = 0., eps_y = 0., eps_z = 0.;
+  for(int tk = 0; tk < Gx.kn; tk++) for(int tj = 0; tj < Gx.jn; tj++)
+    eps_x += -s->u_star[IDX(Gx, Gx.is, tj+DOM_BUF, tk+DOM_BUF)];
+  for(int tk = 0; tk < Gx.kn; tk++) for(int tj = 0; tj < Gx.jn; tj++)
+    eps_x += s->u_star[IDX(Gx, Gx.ie-1, tj+DOM_BUF, tk+DOM_BUF)];
+  eps_x = eps_x * d->dy * d->dz;
+  for(int tk = 0; tk < Gy.kn; tk++) for(int ti = 0; ti < Gy.in; ti++)
+    eps_y += -s->v_star[IDX(Gy, ti+DOM_BUF, Gy.js, tk+DOM_BUF)];
+  for(int tk = 0; tk < Gy.kn; tk++) for(int ti = 0; ti < Gy.in; ti++)
+    eps_y += s->v_star[IDX(Gy, ti+DOM_BUF, Gy.je-1, tk+DOM_BUF)];
+  eps_y = eps_y * d->dx * d->dz;
+  for(int tj = 0; tj < Gz.jn; tj++) for(int ti = 0; ti < Gz.in; ti++)
+    eps_z += -s->w_star[IDX(Gz, ti+DOM_BUF, tj+DOM_BUF, Gz.ks)];
+  for(int tj = 0; tj < Gz.jn; tj++) for(int ti = 0; ti < Gz.in; ti++)
+    eps_z += s->w_star[IDX(Gz, ti+DOM_BUF, tj+DOM_BUF, Gz.ke-1)];
+  eps_z = eps_z * d->dx * d->dy;
+
+  real ex = 0, ey = 0, ez = 0;   /* per-plane increments */
+  int plane[6] = {0, 0, 0, 0, 0, 0};
+  real add[6] = {0, 0, 0, 0, 0, 0};
+  switch(s->out_plane) {
+    case WEST: case EAST: {
+      eps_x = eps_x / d->yl / d->zl;
+      eps_y = eps_y / d->yl / d->zl;
+      eps_z = eps_z / d->yl / d->zl;
+      int f = s->out_plane == WEST ? 0 : 1;
+      plane[f] = 1; add[f] = eps_x + eps_y + eps_z;
+      break; }
+    case SOUTH: case NORTH: {
+      eps_x = eps_x / d->zl / d->xl;
+      eps_y = eps_y / d->zl / d->xl;
+      eps_z = eps_z / d->zl / d->xl;
+      int f = s->out_plane == SOUTH ? 2 : 3;
+      plane[f] = 1; add[f] = eps_x + eps_y + eps_z;
+      break; }
+    case BOTTOM: case TOP: {
+      eps_x = eps_x / d->xl / d->yl;
+      eps_y = eps_y / d->xl / d->yl;
+      eps_z = eps_z / d->xl / d->yl;
+      int f = s->out_plane == BOTTOM ? 4 : 5;
+      plane[f] = 1; add[f] = eps_x + eps_y + eps_z;
+      break; }
+    case HOMOGENEOUS:
+      ex = 0.5 * eps_x / d->yl / d->zl;
+      ey = 0.5 * eps_y / d->zl / d->xl;
+      ez = 0.5 * eps_z / d->xl / d->yl;
+      for(int f = 0; f < 6; f++) plane[f] = 1;
+      add[0] = add[1] = ex; add[2] = add[3] = ey; add[4] = add[5] = ez;
+      break;
+  }
+  /* plane_eps_x_W adds, plane_eps_x_E subtracts, etc. */
+  if(plane[0]) for(int tk = 0; tk < Gx.kn; tk++) for(int tj = 0; tj < Gx.jn; tj++) {
+    int C = IDX(Gx, Gx.is, tj+DOM_BUF, tk+DOM_BUF); s->u_star[C] = s->u_star[C] + add[0]; }
+  if(plane[1]) for(int tk = 0; tk < Gx.kn; tk++) for(int tj = 0; tj < Gx.jn; tj++) {
+    int C = IDX(Gx, Gx.ie-1, tj+DOM_BUF, tk+DOM_BUF); s->u_star[C] = s->u_star[C] - add[1]; }
+  if(plane[2]) for(int tk = 0; tk < Gy.kn; tk++) for(int ti = 0; ti < Gy.in; ti++) {
+    int C = IDX(Gy, ti+DOM_BUF, Gy.js, tk+DOM_BUF); s->v_star[C] = s->v_star[C] + add[2]; }
+  if(plane[3]) for(int tk = 0; tk < Gy.kn; tk++) for(int ti = 0; ti < Gy.in; ti++) {
+    int C = IDX(Gy, ti+DOM_BUF, Gy.je-1, tk+DOM_BUF); s->v_star[C] = s->v_star[C] - add[3]; }
+  if(plane[4]) for(int tj = 0; tj < Gz.jn; tj++) for(int ti = 0; ti < Gz.in; ti++) {
+    int C = IDX(Gz, ti+DOM_BUF, tj+DOM_BUF, Gz.ks); s->w_star[C] = s->w_star[C] + add[4]; }
+  if(plane[5]) for(int tj = 0; tj < Gz.jn; tj++) for(int ti = 0; ti < Gz.in; ti++) {
+    int C = IDX(Gz, ti+DOM_BUF, tj+DOM_BUF, Gz.ke-1); s->w_star[C] = s->w_star[C] - add[5]; }
+}
+
+/* ---------------------------------------------------------------------
+ * Projection: src/bluebottle_kernel.cu:1250-1302, launched by
+ * src/cuda_bluebottle.cu:3422-3497 with ddx = 1/dx and p = phi.
+ * ------------------------------------------------------------------- */
+void orc_project(orc_state *s)
+{
+  const dom_struct *d = &s->dom;
+  const grid_info Gc = d->Gcc, Gx = d->Gfx, Gy = d->Gfy, Gz = d->Gfz;
+  const real ddx = 1. / d->dx, ddy = 1. / d->dy, ddz = 1. / d->dz;
+  const real dt = s->dt, rho_f = s->rho_f;
+  const real *phi = s->phi;
+  for(int k = DOM_BUF; k < Gx.ke; k++) for(int j = DOM_BUF; j < Gx.je; j++)
+  for(int i = Gx.is; i < Gx.ie; i++) {
+    int C = IDX(Gx, i, j, k);
+    real gradPhi = abs(s->flag_u[C]) * ddx
+      * (phi[IDX(Gc, i, j, k)] - phi[IDX(Gc, i-1, j, k)]);
+    s->u[C] = (s->u_star[C] - dt / rho_f * gradPhi);
+  }
+  for(int k = DOM_BUF; k < Gy.ke; k++) for(int j = Gy.js; j < Gy.je; j++)
+  for(int i = DOM_BUF; i < Gy.ie; i++) {
+    int C = IDX(Gy, i, j, k);
+    real gradPhi = abs(s->flag_v[C]) * ddy
+      * (phi[IDX(Gc, i, j, k)] - phi[IDX(Gc, i, j-1, k)]);
+    s->v[C] = (s->v_star[C] - dt / rho_f * gradPhi);
+  }
+  for(int k = Gz.ks; k < Gz.ke; k++) for(int j = DOM_BUF; j < Gz.je; j++)
+  for(int i = DOM_BUF; i < Gz.ie; i++) {
+    int C = IDX(Gz, i, j, k);
+    real gradPhi = abs(s->flag_w[C]) * ddz
+      * (phi[IDX(Gc, i, j, k)] - phi[IDX(Gc, i, j, k-1)]);
+    s->w[C] = (s->w_star[C] - dt / rho_f * gradPhi);
+  }
+}
+
+/* Upper bound (exclusive) of the j or k index that forcing_add_c_const
+ * reaches for a launch sized on the interior extent n (16-wide blocks):
+ * src/cuda_bluebottle.cu:3631-3669 with src/bluebottle_kernel.cu:2042. */
+static int c_const_limit(int n, int nb)
+{
+  int th = n < MAX_THREADS_DIM ? n : MAX_THREADS_DIM;
+  int blocks = (int) ceil((real) n / (real) th);
+  int lim = blocks * th + DOM_BUF;
+  return lim < nb ? lim : nb;
+}
+
+/* ---------------------------------------------------------------------
+ * Pressure update: update_p (src/bluebottle_kernel.cu:1326-1338), then the
+ * mean over the interior (copy_p_noghost + thrust::reduce, summed here in
+ * array order) is subtracted by forcing_add_c_const
+ * (src/cuda_bluebottle.cu:3616-3671).
+ * ------------------------------------------------------------------- */
+void orc_update_p(orc_state *s)
+{
+  const grid_info Gc = s->dom.Gcc;
+  for(int k = Gc.ks; k < Gc.ke; k++) for(int j = Gc.js; j < Gc.je; j++)
+  for(int i = Gc.is; i < Gc.ie; i++) {
+    int C = IDX(Gc, i, j, k);
+    s->p[C] = (s->phase[C] < 0) * (s->p0[C] + s->phi[C]);
+  }
+  real sum = 0.;
+  for(int k = Gc.ks; k < Gc.ke; k++) for(int j = Gc.js; j < Gc.je; j++)
+  for(int i = Gc.is; i < Gc.ie; i++) sum += s->p[IDX(Gc, i, j, k)];
+  real pmean = sum / Gc.s3;
+  real val = -pmean;
+  int jl = c_const_limit(Gc.jn, Gc.jnb), kl = c_const_limit(Gc.kn, Gc.knb);
+  for(int k = DOM_BUF; k < kl; k++) for(int j = DOM_BUF; j < jl; j++)
+  for(int i = Gc.isb; i < Gc.ieb; i++) s->p[IDX(Gc, i, j, k)] += val;
+}
+
+/* ---------------------------------------------------------------------
+ * Time-step size: src/cuda_bluebottle.cu:3500-3577 (explicit build).  The
+ * maxima are of the signed velocities over the whole ghosted arrays,
+ * starting from 0.
+ * ------------------------------------------------------------------- */
+real orc_find_dt(orc_state *s)
+{
+  const dom_struct *d = &s->dom;
+  real umax = 0., vmax = 0., wmax = 0.;
+  for(int n = 0; n < d->Gfx.s3b; n++) if(s->u[n] > umax) umax = s->u[n];
+  for(int n = 0; n < d->Gfy.s3b; n++) if(s->v[n] > vmax) vmax = s->v[n];
+  for(int n = 0; n < d->Gfz.s3b; n++) if(s->w[n] > wmax) wmax = s->w[n];
+  real tmp = umax / d->dx + 2.*s->nu/d->dx/d->dx;
+  tmp += vmax / d->dy + 2.*s->nu/d->dy/d->dy;
+  tmp += wmax / d->dz + 2.*s->nu/d->dz/d->dz;
+  real dts = s->CFL / tmp;
+  real mn = FLT_MAX;
+  if(dts < mn) mn = dts;
+  return mn;
+}
+
+/* src/cuda_bluebottle.cu:3580-3613 */
+void orc_store_u(orc_state *s)
+{
+  const dom_struct *d = &s->dom;
+  size_t fx = d->Gfx.s3b * sizeof(real), fy = d->Gfy.s3b * sizeof(real);
+  size_t fz = d->Gfz.s3b * sizeof(real), cc = d->Gcc.s3b * sizeof(real);
+  memcpy(s->conv0_u, s->conv_u, fx);
+  memcpy(s->conv0_v, s->conv_v, fy);
+  memcpy(s->conv0_w, s->conv_w, fz);
+  memcpy(s->diff0_u, s->diff_u, fx);
+  memcpy(s->diff0_v, s->diff_v, fy);
+  memcpy(s->diff0_w, s->diff_w, fz);
+  memcpy(s->p0, s->p, cc);
+  memcpy(s->u0, s->u, fx);
+  memcpy(s->v0, s->v, fy);
+  memcpy(s->w0, s->w, fz);
+}
+
+/* Linear ramp of one driving component (src/cuda_bluebottle.cu:3727-3760). */
+static real ramp(real delta, real a, real m)
+{
+  if(a == 0) return m;
+  if(fabs(delta * a) > fabs(m)) return m;
+  return delta * a;
+}
+
+/* src/cuda_bluebottle.cu:3674-3814 */
+void orc_compute_forcing(orc_state *s)
+{
+  const dom_struct *d = &s->dom;
+  real delta = s->ttime - s->p_tDelay;
+  int pid = (s->Kp > 0 || s->Ki > 0 || s->Kd > 0);
+  if(delta >= 0) {
+    s->gradP.x = ramp(delta, s->gradP.xa, s->gradP.xm);
+    s->gradP.y = ramp(delta, s->gradP.ya, s->gradP.ym);
+    if(!pid) s->gradP.z = ramp(delta, s->gradP.za, s->gradP.zm);
+  }
+  delta = s->ttime - s->g_tDelay;
+  if(delta >= 0) {
+    s->g.x = ramp(delta, s->g.xa, s->g.xm);
+    s->g.y = ramp(delta, s->g.ya, s->g.ym);
+    s->g.z = ramp(delta, s->g.za, s->g.zm);
+  }
+  delta = s->ttime - s->p_tDelay;
+  if(delta >= 0 && pid) {
+    real acc_z = 0, volp = 0, massp = 0;
+    for(int i = 0; i < s->nparts; i++) {
+      const part_struct *P = &s->parts[i];
+      volp += P->r*P->r*P->r;
+      massp += P->rho*P->r*P->r*P->r;
+      acc_z += P->wdot;
+    }
+    volp *= 4./3.*PI;
+    massp *= 4./3.*PI;
+    real volfrac = volp / (d->xl * d->yl * d->zl);
+    real rho_avg = massp/volp*volfrac + s->rho_f*(1.-volfrac);
+    acc_z /= s->nparts;
+    s->pid_int = s->pid_int + acc_z*s->dt;
+    s->gradP.z = s->gradP.z
+      + (s->Kp*acc_z + s->Ki*(s->pid_int)/s->ttime
+      + (s->Kd)*(acc_z-s->pid_back))*rho_avg;
+    s->pid_back = acc_z;
+  }
+  real fx = 0. + -s->gradP.x / s->rho_f;
+  real fy = 0. + -s->gradP.y / s->rho_f;
+  real fz = 0. + -s->gradP.z / s->rho_f;
+  for(int n = 0; n < d->Gfx.s3b; n++) s->f_x[n] = fx;
+  for(int n = 0; n < d->Gfy.s3b; n++) s->f_y[n] = fy;
+  for(int n = 0; n < d->Gfz.s3b; n++) s->f_z[n] = fz;
+}
