@@ -1,0 +1,309 @@
+"""Time steps per second of the Bluebottle step on one MI355X.
+
+Workload: particle sedimentation in a 256^3 triply-periodic box with 1000
+spheres of radius 6 dx (volume fraction 5.4 %), density ratio 2, Re_p = 10,
+Lamb order L = 4 with the 26-node rule, pressure Poisson solve at relative
+residual 1e-8 (Jacobi-preconditioned BiCGSTAB), synthetic seeded particle
+positions, quiescent start.  One step is the full src/bluebottle.c:598-711
+sequence (forcing, cages, the Lamb sub-iteration loop of predictor /
+Poisson solve / projection / pressure update / particle BCs / quadrature,
+particle motion, history shift, next dt), driven through the C ABI.
+
+Both solver forms run on the same case, one after the other: the plain
+FP64 BiCGSTAB and the mixed form (FP32 BiCGSTAB inside FP64 iterative
+refinement).  `value` is the mixed form's rate.  Per-kernel durations come
+from HIP events recorded on the library stream around every launch inside
+the timed region; the roofline entry is the kernel with the largest total
+time, priced at its algorithmic bytes per launch.
+
+The CPU baseline times the serial FP64 oracle on this host over a bounded
+sample (one pass of every stage of a Lamb sub-iteration plus a fixed number
+of BiCGSTAB iterations at full size) and extrapolates to a step with the
+Lamb sub-iteration and Krylov iteration counts the GPU's FP64 run measured.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("timesteps/s @256³+1000 spheres, learned+mixed vs plain FP64; "
+          "HBM GB/s")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(n, xn, yn, zn):
+    """Algorithmic bytes per launch for the single-launch kernel kinds:
+    every input and output array once (FP64 8 B, FP32 4 B, int32 4 B,
+    packed operator flags 1 B)."""
+    nf = (xn + 1) * yn * zn + xn * (yn + 1) * zn + xn * yn * (zn + 1)
+    return {
+        # 13 FP64 reads + 1 int, 9 FP64 writes per face (3 components)
+        "u_star": 180 * n,
+        # u*, v*, w*, phase, phase_shell, flag_u/v/w, phi -> rhs, x, flags
+        "pp_setup": (24 + 8 + 12 + 8 + 8 + 8 + 1) * n,
+        # p, r*, flags -> v
+        "spmv_p": (3 * 8 + 1) * n, "spmv_p32": (3 * 4 + 1) * n,
+        # r, v, flags -> t
+        "spmv_s": (3 * 8 + 1) * n, "spmv_s32": (3 * 4 + 1) * n,
+        # x, p, r, v, t, r*, flags -> x, r
+        "update_xr": (8 * 8 + 1) * n, "update_xr32": (8 * 4 + 1) * n,
+        # r, p, v -> p
+        "update_p": 4 * 8 * n, "update_p32": 4 * 4 * n,
+        # b, x, flags -> r (FP64)
+        "resid64": (3 * 8 + 1) * n,
+        # x, e -> x
+        "correct64": (2 * 8 + 4) * n,
+        # u*, v*, w*, phi, flag_u/v/w -> u, v, w
+        "project": 68 * n,
+        # p0, phi, phase -> p (fused sum and subtract)
+        "update_pressure": 44 * n,
+        "find_dt": 8 * nf,
+    }
+
+
+def run_mode(s, mode, steps, warmup, runtime):
+    host, hip = s.host, s.hip
+    s.set_solver(mode)
+    for i in range(warmup):
+        t0 = time.perf_counter()
+        it = s.step()
+        hip.bb_synchronize()
+        log("mode %d warmup step %d: %d Lamb iterations, %.3f s" % (
+            mode, i, it, time.perf_counter() - t0))
+    import ctypes as C
+    tot = C.c_long.in_dll(host, "bb_rec_total_iters")
+    calls = C.c_int.in_dll(host, "bb_rec_calls")
+    it0, c0 = tot.value, calls.value
+    hip.bb_prof_reset()
+    hip.bb_prof_enable(1)
+    lamb_iters = 0
+    hip.bb_synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        lamb_iters += s.step()
+        log("mode %d timed step %d done (%.3f s elapsed)" % (
+            mode, i, time.perf_counter() - t0))
+    hip.bb_synchronize()
+    elapsed = time.perf_counter() - t0
+    hip.bb_prof_enable(0)
+    prof = {}
+    for k, name in enumerate(runtime.PROF_KINDS):
+        ms = C.c_double()
+        cnt = C.c_long()
+        hip.bb_prof_read(k, C.byref(ms), C.byref(cnt))
+        if cnt.value:
+            prof[name] = (ms.value, cnt.value)
+    return {
+        "seconds": elapsed,
+        "ms_per_step": 1e3 * elapsed / steps,
+        "timesteps_per_s": steps / elapsed,
+        "lamb_iters_per_step": lamb_iters / steps,
+        "krylov_iters_per_solve": (tot.value - it0) / max(calls.value - c0, 1),
+        "solves": calls.value - c0,
+        "krylov_iters": tot.value - it0,
+        "prof": prof,
+    }
+
+
+def roofline(prof, nbytes):
+    best = None
+    for name, (ms, cnt) in prof.items():
+        if name not in nbytes:
+            continue
+        if best is None or ms > best[1]:
+            best = (name, ms, cnt)
+    if best is None:
+        return None, {}
+    table = {}
+    for name, (ms, cnt) in prof.items():
+        if name in nbytes:
+            avg = ms / cnt
+            gbs = nbytes[name] / (avg * 1e-3) / 1e9
+            table[name] = {"launches": cnt, "avg_us": round(avg * 1e3, 2),
+                           "total_ms": round(ms, 2),
+                           "GB/s": round(gbs, 1),
+                           "frac": round(gbs / HBM_PEAK_GBS, 3)}
+    name, ms, cnt = best
+    avg = ms / cnt
+    achieved = nbytes[name] / (avg * 1e-3) / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_us": round(avg * 1e3, 2),
+            "bytes_per_launch": nbytes[name]}, table
+
+
+def cpu_baseline(s, fp64, krylov_sample):
+    """Serial FP64 oracle on this host, bounded sample, extrapolated."""
+    import ctypes as C
+    from oracle.pyoracle import OracleState
+    s.pull()
+    o = OracleState.from_sim(s)
+    lib = o.lib
+    nparts = o.nparts
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+
+    # one Lamb sub-iteration without the solve
+    stage = 0.0
+    stage += timed(lambda: o.call("U_star_2"))
+    if nparts:
+        stage += timed(lambda: o.call("part_BC_star"))
+    stage += timed(lambda: o.call("dom_BC_star"))
+    stage += timed(lambda: o.call("solvability"))
+    if nparts:
+        stage += timed(lambda: o.call("part_BC_star"))
+    stage += timed(lambda: o.call("dom_BC_star"))
+    # rhs (and the particle mask), then a fixed number of iterations
+    stage += timed(lambda: o.call("PP_rhs"))
+    if nparts:
+        stage += timed(lambda: o.call("part_BC_p"))
+    n = o.s.dom.Gcc.s3
+    import numpy as np
+    b = np.array(o.array("rhs_p")[:n])
+    nrm = float(np.sqrt(np.dot(b, b))) or 1.0
+    b /= nrm
+    x = np.zeros(n)
+    resid = C.c_double()
+    conv = C.c_int()
+    t_solve = timed(lambda: lib.orc_bicgstab(
+        o.ptr, b.ctypes.data_as(C.POINTER(C.c_double)),
+        x.ctypes.data_as(C.POINTER(C.c_double)), krylov_sample, 0.0,
+        C.byref(resid), C.byref(conv)))
+    per_iter = t_solve / krylov_sample
+    stage += timed(lambda: o.call("dom_BC_phi"))
+    stage += timed(lambda: o.call("project"))
+    if nparts:
+        stage += timed(lambda: o.call("part_BC"))
+    stage += timed(lambda: o.call("dom_BC"))
+    stage += timed(lambda: o.call("update_p"))
+    if nparts:
+        stage += timed(lambda: o.call("part_BC"))
+        stage += timed(lambda: o.call("part_p_fill"))
+    stage += timed(lambda: o.call("dom_BC_p"))
+    if nparts:
+        stage += timed(lambda: o.call("move_parts_sub"))
+        stage += timed(lambda: o.call("Lamb"))
+        stage += timed(lambda: lib.orc_lamb_err(o.ptr))
+    outer = 0.0
+    outer += timed(lambda: o.call("build_cages"))
+    if nparts:
+        outer += timed(lambda: o.call("move_parts"))
+        outer += timed(lambda: o.call("parts_internal"))
+    outer += timed(lambda: o.call("dom_BC"))
+    outer += timed(lambda: o.call("store_u"))
+    if nparts:
+        outer += timed(lambda: o.call("store_coeffs"))
+    outer += timed(lambda: lib.orc_find_dt(o.ptr))
+    li = fp64["lamb_iters_per_step"]
+    ki = fp64["krylov_iters_per_solve"]
+    step_s = outer + li * (stage + ki * per_iter)
+    sample_s = stage + outer + t_solve
+    del o
+    return {
+        "value": round(1.0 / step_s, 6), "unit": "timesteps/s", "cores": 1,
+        "kind": "port",
+        "sample": ("serial FP64 oracle on the full 256^3 + 1000-sphere state: "
+                   "one pass of each sub-iteration stage (%.1f s), the step's "
+                   "outer stages (%.1f s) and %d BiCGSTAB iterations "
+                   "(%.2f s/iteration), %.1f s measured in all; extrapolated "
+                   "to %.2f Lamb sub-iterations x %.1f Krylov iterations per "
+                   "step as the GPU FP64 run measured" % (
+                       stage, outer, krylov_sample, per_iter, sample_s, li,
+                       ki)),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--nparts", type=int, default=1000)
+    ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--krylov-sample", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not "
+              "supported (the multi-GPU decomposition is a design, not an "
+              "implementation)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from bluebottle_amd import configs, runtime, sim
+    case = configs.config3(n=args.n, nparts=args.nparts)
+    log("setting up %d^3 with %d spheres" % (args.n, args.nparts))
+    t0 = time.perf_counter()
+    s = sim.Simulation(case)
+    s.to_device()
+    s.prepare()
+    s.hip.bb_synchronize()
+    log("set-up done in %.1f s on %s" % (time.perf_counter() - t0,
+                                        s.hip.bb_device_name().decode()))
+    xn, yn, zn = case.n
+    ncell = xn * yn * zn
+    nbytes = kernel_bytes(ncell, xn, yn, zn)
+    modes = {}
+    names = {"fp64": 0, "mixed": 1}
+    for m in args.modes.split(","):
+        modes[m] = run_mode(s, names[m], args.steps, args.warmup, runtime)
+        log("%s: %.4f timesteps/s, %.2f Lamb it/step, %.1f Krylov it/solve" % (
+            m, modes[m]["timesteps_per_s"], modes[m]["lamb_iters_per_step"],
+            modes[m]["krylov_iters_per_solve"]))
+    head = "mixed" if "mixed" in modes else list(modes)[0]
+    H = modes[head]
+    roof, table = roofline(H["prof"], nbytes)
+    cpu = None
+    if not args.no_cpu and "fp64" in modes:
+        log("timing the CPU oracle sample")
+        cpu = cpu_baseline(s, modes["fp64"], args.krylov_sample)
+        log("cpu sample done")
+    out = {
+        "metric": METRIC,
+        "value": round(H["timesteps_per_s"], 5),
+        "unit": "timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(H["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32" if head == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "sedimentation %dx%dx%d periodic, %d spheres r=6dx, "
+                        "rho_p/rho_f=2, Re_p=10, L=4, 26 nodes, Poisson rel "
+                        "tol 1e-8" % (xn, yn, zn, args.nparts),
+            "solver": head + " BiCGSTAB, Jacobi preconditioner",
+            "lamb_max_iter": case.lamb_max_iter,
+        },
+        "modes": {m: {k: (round(v, 5) if isinstance(v, float) else v)
+                      for k, v in d.items() if k != "prof"}
+                  for m, d in modes.items()},
+        "roofline": roof,
+        "kernels": table,
+        "cpu_baseline": cpu,
+    }
+    if "fp64" in modes and "mixed" in modes:
+        out["mixed_over_fp64"] = round(
+            modes["mixed"]["timesteps_per_s"] / modes["fp64"]["timesteps_per_s"],
+            3)
+    s.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

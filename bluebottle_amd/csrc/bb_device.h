@@ -95,6 +95,7 @@ void bb_ensure_workspace(void);
 void bb_free_workspace(void);
 /* Particle workspace sized for nparts and the current grid. */
 void bb_ensure_particle_workspace(void);
+void bb_invalidate_face_lists(void);
 
 /* ---- kernel timing (enabled by bb_prof_enable) ---- */
 #define BB_PROF_MAX_KINDS 32
@@ -107,6 +108,9 @@ enum {
   BB_K_PROJECT, BB_K_UPDATEP, BB_K_FINDDT, BB_K_CAGE, BB_K_PARTBC,
   BB_K_QUAD, BB_K_MOVE, BB_K_MISC, BB_K_NKINDS
 };
+static_assert((int) BB_K_NKINDS == (int) BB_PROF_NKINDS &&
+  (int) BB_K_SPMV_S32 == (int) BB_PROF_SPMV_S32 &&
+  (int) BB_K_MISC == (int) BB_PROF_MISC, "kernel kinds follow bluebottle.h");
 
 /* ---- block reductions over 256 threads (4 wavefronts of 64) ---- */
 __device__ inline double bb_wave_sum(double v)

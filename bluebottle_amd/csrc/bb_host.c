@@ -83,12 +83,14 @@ char bb_record_dir[FILE_NAME_SIZE] = "";
 int bb_rec_niter = 0;
 real bb_rec_resid = 0.;
 int bb_rec_calls = 0;
+long bb_rec_total_iters = 0;
 
 void recorder_bicgstab(char *name, int niter, real resid)
 {
   bb_rec_niter = niter;
   bb_rec_resid = resid;
   bb_rec_calls++;
+  bb_rec_total_iters += niter;
   if(bb_record_dir[0] == '\0') return;
   char path[2 * FILE_NAME_SIZE];
   snprintf(path, sizeof(path), "%s/%s", bb_record_dir, name);

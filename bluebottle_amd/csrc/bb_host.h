@@ -19,6 +19,7 @@ extern char bb_record_dir[FILE_NAME_SIZE];
 extern int bb_rec_niter;
 extern real bb_rec_resid;
 extern int bb_rec_calls;
+extern long bb_rec_total_iters;
 
 /* Grid set-up and zeroed host arrays (src/domain.c:1591-2347).  The caller
  * sets rho_f, nu, mu, CFL and the BC/driving globals before calling, and

@@ -80,7 +80,13 @@ void bb_ensure_workspace(void)
   if(ws->n == n && ws->x) return;
   bb_free_workspace();
   ws->n = n;
-  ws->nblk = 4096;
+  /* one partial slot group per 64 x 4 x 16 Poisson tile, and at least the
+   * 4096 slots the grid-stride reductions use */
+  {
+    long tiles = (long) ((dom[0].xn + 63) / 64) * ((dom[0].yn + 3) / 4)
+      * ((dom[0].zn + 15) / 16);
+    ws->nblk = (int) (tiles > 4096 ? tiles : 4096);
+  }
   ws->x = (double *) dmalloc(n * sizeof(double));
   ws->r = (double *) dmalloc(n * sizeof(double));
   ws->rs = (double *) dmalloc(n * sizeof(double));
@@ -279,6 +285,7 @@ extern "C" void cuda_part_push(void)
   BB_CHECK(hipMemcpyAsync(_binDom, &binDom, sizeof(dom_struct),
     hipMemcpyHostToDevice, g_bb.stream));
   BB_CHECK(hipStreamSynchronize(g_bb.stream));
+  bb_invalidate_face_lists();
 }
 
 extern "C" void cuda_part_pull(void)

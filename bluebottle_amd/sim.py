@@ -273,6 +273,8 @@ class Simulation:
         return st
 
     def close(self):
+        if Simulation._alive is not self:
+            return
         if self.device:
             self.hip.cuda_part_free()
             self.hip.cuda_dom_free()

@@ -355,6 +355,43 @@ void bb_get_solve_stats(bb_solve_stats *out);
 /* Returns a short string naming the HIP device the library runs on. */
 const char *bb_device_name(void);
 
+/* Non-convergence of the pressure solve prints the reference's message and
+ * exits (on = 1, the default, as src/cuda_bicgstab.cu:1050-1056 does);
+ * on = 0 keeps the message and returns. */
+void bb_set_exit_on_nonconvergence(int on);
+
+/* Blocks until all work the library queued has finished. */
+void bb_synchronize(void);
+
+/* The HIP stream (hipStream_t) all entry points run on, as void *. */
+void *bb_stream(void);
+
+/* Krylov convergence is checked on the host every `n` iterations (the
+ * device keeps iterating scalars; iterations past convergence are no-ops). */
+void bb_set_poll_interval(int n);
+
+/* Mixed mode: inner FP32 relative tolerance, inner and outer iteration
+ * caps (values <= 0 leave a parameter unchanged). */
+void bb_set_mixed_params(real inner_tol, int inner_max, int outer_max);
+
+/* Per-kernel event timing on the library stream.  Kinds index the kernel
+ * families (predictor, BCs, Poisson set-up, the four BiCGSTAB sweeps in
+ * FP64 and FP32, refinement residual and correction, projection, pressure
+ * update, dt, cages, particle BCs, quadrature, particle motion, misc), in
+ * that order, see BB_PROF_* below. */
+enum {
+  BB_PROF_USTAR = 0, BB_PROF_BC, BB_PROF_PP_SETUP, BB_PROF_PACK,
+  BB_PROF_SPMV_P, BB_PROF_SPMV_S, BB_PROF_UPDATE_XR, BB_PROF_UPDATE_P,
+  BB_PROF_FINAL, BB_PROF_RESID64, BB_PROF_CORRECT64, BB_PROF_SPMV_P32,
+  BB_PROF_SPMV_S32, BB_PROF_UPDATE_XR32, BB_PROF_UPDATE_P32,
+  BB_PROF_PROJECT, BB_PROF_UPDATE_PRESSURE, BB_PROF_FIND_DT, BB_PROF_CAGES,
+  BB_PROF_PART_BC, BB_PROF_QUADRATURE, BB_PROF_MOVE_PARTS, BB_PROF_MISC,
+  BB_PROF_NKINDS
+};
+void bb_prof_enable(int on);
+void bb_prof_read(int kind, real *ms, long *count);
+void bb_prof_reset(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,273 @@
+/*
+ * Host side of the CPU emulation of HIP (test infrastructure): device
+ * allocations as separate heap blocks (so AddressSanitizer bounds every
+ * buffer), a registry of them for pointer-argument checks, synchronous
+ * copies, and the block executor: one block at a time, its threads run
+ * concurrently on a persistent pool with a real barrier for __syncthreads.
+ */
+#include <hip/hip_runtime.h>
+
+#include <barrier>
+#include <condition_variable>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+thread_local dim3 threadIdx;
+thread_local dim3 blockIdx;
+dim3 blockDim;
+dim3 gridDim;
+
+namespace {
+
+std::mutex g_reg_mu;
+std::map<uintptr_t, size_t> g_reg;   /* base -> size */
+
+void reg_add(void *p, size_t n)
+{
+  std::lock_guard<std::mutex> l(g_reg_mu);
+  g_reg[(uintptr_t) p] = n;
+}
+
+bool reg_del(void *p)
+{
+  std::lock_guard<std::mutex> l(g_reg_mu);
+  return g_reg.erase((uintptr_t) p) == 1;
+}
+
+const char *g_kernel = "";
+
+/* per-block state shared by the pool threads */
+struct BlockState {
+  std::unique_ptr<std::barrier<>> bar;
+  std::vector<double> xd;
+  std::vector<float> xf;
+  std::vector<int> xi;
+  int nthreads = 0;
+};
+BlockState g_blk;
+
+int flat_tid(void)
+{
+  return (int) (threadIdx.x + threadIdx.y * blockDim.x
+    + threadIdx.z * blockDim.x * blockDim.y);
+}
+
+class Pool {
+ public:
+  void run(int k, const std::function<void(int)> &job)
+  {
+    ensure(k);
+    {
+      std::unique_lock<std::mutex> l(mu_);
+      job_ = &job;
+      active_ = k;
+      remaining_ = k;
+      gen_++;
+    }
+    cv_.notify_all();
+    std::unique_lock<std::mutex> l(mu_);
+    done_.wait(l, [&] { return remaining_ == 0; });
+  }
+
+ private:
+  void ensure(int k)
+  {
+    while((int) th_.size() < k) {
+      int id = (int) th_.size();
+      th_.emplace_back([this, id] { worker(id); });
+      th_.back().detach();
+    }
+  }
+  void worker(int id)
+  {
+    uint64_t seen = 0;
+    for(;;) {
+      std::unique_lock<std::mutex> l(mu_);
+      cv_.wait(l, [&] { return gen_ != seen; });
+      seen = gen_;
+      if(id >= active_) continue;
+      const std::function<void(int)> *j = job_;
+      l.unlock();
+      (*j)(id);
+      l.lock();
+      if(--remaining_ == 0) done_.notify_all();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  uint64_t gen_ = 0;
+  int active_ = 0, remaining_ = 0;
+  const std::function<void(int)> *job_ = nullptr;
+};
+
+Pool *g_pool = nullptr;
+
+template<typename T>
+T shfl(std::vector<T> &buf, T v, unsigned delta, int width)
+{
+  int tid = flat_tid();
+  buf[tid] = v;
+  g_blk.bar->arrive_and_wait();
+  int lane = tid % width;
+  int base = tid - lane;
+  T out = v;
+  if(lane + (int) delta < width && base + lane + (int) delta < g_blk.nthreads)
+    out = buf[base + lane + delta];
+  g_blk.bar->arrive_and_wait();
+  return out;
+}
+
+}  // namespace
+
+namespace bb_emu {
+
+void syncthreads(void) { g_blk.bar->arrive_and_wait(); }
+
+double shfl_down_d(double v, unsigned d, int w) { return shfl(g_blk.xd, v, d, w); }
+float shfl_down_f(float v, unsigned d, int w) { return shfl(g_blk.xf, v, d, w); }
+int shfl_down_i(int v, unsigned d, int w) { return shfl(g_blk.xi, v, d, w); }
+
+bool is_device_range(const void *p, size_t bytes)
+{
+  if(p == nullptr) return true;
+  std::lock_guard<std::mutex> l(g_reg_mu);
+  uintptr_t a = (uintptr_t) p;
+  auto it = g_reg.upper_bound(a);
+  if(it == g_reg.begin()) return false;
+  --it;
+  return a >= it->first && a + bytes <= it->first + (it->second ? it->second : 1);
+}
+
+void report_bad_pointer(const char *kernel, int arg, const void *p)
+{
+  fprintf(stderr, "bb_emu: kernel %s argument %d = %p is not a device "
+    "allocation\n", kernel, arg, p);
+  fflush(stderr);
+  abort();
+}
+
+void run_grid(const char *name, dim3 g, dim3 b, const std::function<void()> &body)
+{
+  if(!g_pool) g_pool = new Pool();
+  long nth = (long) b.x * b.y * b.z;
+  if(nth <= 0 || nth > 1024 || g.x == 0 || g.y == 0 || g.z == 0
+     || g.y > 65535 || g.z > 65535) {
+    fprintf(stderr, "bb_emu: kernel %s has an invalid launch shape "
+      "grid (%u,%u,%u) block (%u,%u,%u)\n", name, g.x, g.y, g.z, b.x, b.y, b.z);
+    fflush(stderr);
+    abort();
+  }
+  g_kernel = name;
+  gridDim = g;
+  blockDim = b;
+  g_blk.nthreads = (int) nth;
+  g_blk.xd.assign(nth, 0.);
+  g_blk.xf.assign(nth, 0.f);
+  g_blk.xi.assign(nth, 0);
+  for(unsigned bz = 0; bz < g.z; bz++)
+  for(unsigned by = 0; by < g.y; by++)
+  for(unsigned bx = 0; bx < g.x; bx++) {
+    g_blk.bar.reset(new std::barrier<>(nth));
+    dim3 bi(bx, by, bz);
+    std::function<void(int)> job = [&](int t) {
+      blockIdx = bi;
+      threadIdx = dim3(t % b.x, (t / b.x) % b.y, t / (b.x * b.y));
+      body();
+      g_blk.bar->arrive_and_drop();
+    };
+    g_pool->run((int) nth, job);
+  }
+}
+
+}  // namespace bb_emu
+
+/* ---- runtime API ---- */
+hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
+hipError_t hipSetDevice(int) { return hipSuccess; }
+hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int)
+{
+  memset(p, 0, sizeof(*p));
+  snprintf(p->name, sizeof(p->name), "CPU emulation (test build)");
+  snprintf(p->gcnArchName, sizeof(p->gcnArchName), "emu");
+  p->multiProcessorCount = 1;
+  return hipSuccess;
+}
+hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned)
+{
+  *s = (hipStream_t) 0x1;
+  return hipSuccess;
+}
+hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+hipError_t hipDeviceSynchronize(void) { return hipSuccess; }
+
+hipError_t hipMalloc(void **p, size_t n)
+{
+  *p = malloc(n ? n : 1);
+  if(!*p) return hipErrorOutOfMemory;
+  /* fill with a signalling pattern: reads of never-written device memory
+   * show up as NaN instead of zero */
+  memset(*p, 0xff, n);
+  reg_add(*p, n);
+  return hipSuccess;
+}
+hipError_t hipFree(void *p)
+{
+  if(!p) return hipSuccess;
+  if(!reg_del(p)) {
+    fprintf(stderr, "bb_emu: hipFree of %p, not a device allocation\n", p);
+    abort();
+  }
+  free(p);
+  return hipSuccess;
+}
+hipError_t hipHostMalloc(void **p, size_t n, unsigned)
+{
+  *p = malloc(n ? n : 1);
+  reg_add(*p, n);
+  return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+hipError_t hipHostFree(void *p) { reg_del(p); free(p); return hipSuccess; }
+
+static void check_copy(void *d, const void *s, size_t n, hipMemcpyKind k)
+{
+  bool dd = k == hipMemcpyHostToDevice || k == hipMemcpyDeviceToDevice;
+  bool sd = k == hipMemcpyDeviceToHost || k == hipMemcpyDeviceToDevice;
+  if((dd && !bb_emu::is_device_range(d, n)) || (sd && !bb_emu::is_device_range(s, n))) {
+    fprintf(stderr, "bb_emu: copy of %zu bytes kind %d between %p and %p is "
+      "out of a device allocation\n", n, (int) k, s, d);
+    fflush(stderr);
+    abort();
+  }
+}
+
+hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind k)
+{
+  check_copy(d, s, n, k);
+  memmove(d, s, n);
+  return hipSuccess;
+}
+hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind k,
+  hipStream_t)
+{
+  return hipMemcpy(d, s, n, k);
+}
+hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t)
+{
+  if(!bb_emu::is_device_range(d, n)) {
+    fprintf(stderr, "bb_emu: memset of %zu bytes at %p is out of a device "
+      "allocation\n", n, d);
+    abort();
+  }
+  memset(d, v, n);
+  return hipSuccess;
+}
+struct bb_emu_event { int dummy; };
+hipError_t hipEventCreate(hipEvent_t *e) { *e = new bb_emu_event(); return hipSuccess; }
+hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0.f; return hipSuccess; }
+hipError_t hipGetLastError(void) { return hipSuccess; }
+const char *hipGetErrorString(hipError_t) { return "emulated error"; }

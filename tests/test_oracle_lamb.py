@@ -1,0 +1,224 @@
+"""Pins of the oracle's Lamb-solution pieces against mathematics that fixes
+them (no reference execution is possible: the reference is CUDA).
+
+  * the 26-node Lebedev rule (src/cuda_quadrature.cu:86-157) integrates
+    every monomial x^a y^b z^c of degree <= 7 over the unit sphere exactly
+    (up to the 1e-10 pole nudge DIV_ST), and its weights sum to 4 pi;
+  * P_nm (src/quadrature_kernel.cu:316-375) is the associated Legendre
+    function with the Condon-Shortley phase; N_nm is the spherical-harmonic
+    normalisation;
+  * xyz2rtp (src/quadrature_kernel.cu:305-314) inverts the spherical map;
+  * the whole fit (interpolation, inner products, closed-form solve,
+    forces: src/quadrature_kernel.cu:95-576) reproduces two exact Stokes
+    solutions around a sphere: uniform flow (only n = 1, m = 0 non-zero,
+    drag 6 pi mu a W) and pure straining flow (only n = 2, zero force),
+    and part_BC (src/particle_kernel.cu:535-838) evaluates them back at the
+    flagged faces.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+from scipy.special import gamma, lpmv
+
+from bluebottle_amd import sim
+from oracle.pyoracle import OracleState, load
+
+
+def sphere_moment(a, b, c):
+    """Integral of x^a y^b z^c over the unit sphere."""
+    if a % 2 or b % 2 or c % 2:
+        return 0.0
+    return 2.0 * gamma((a + 1) / 2) * gamma((b + 1) / 2) * gamma((c + 1) / 2) \
+        / gamma((a + b + c + 3) / 2)
+
+
+def nodes():
+    lib = load()
+    t = (C.c_double * 26)()
+    p = (C.c_double * 26)()
+    w = (C.c_double * 26)()
+    lib.orc_lebedev_nodes(t, p, w)
+    return np.array(t[:]), np.array(p[:]), np.array(w[:])
+
+
+def test_lebedev_weights_sum_to_4pi():
+    _, _, w = nodes()
+    assert abs(w.sum() - 4 * math.pi) < 1e-13
+    # the three literal weights are 4 pi/21, 16 pi/105, 9 pi/70
+    assert set(np.round(w, 15)) == {0.598398600683775, 0.478718880547015,
+                                    0.403919055461543}
+    assert abs(0.598398600683775 - 4 * math.pi / 21) < 1e-14
+    assert abs(0.478718880547015 - 16 * math.pi / 105) < 1e-14
+    assert abs(0.403919055461543 - 9 * math.pi / 70) < 1e-14
+
+
+def test_lebedev_exact_to_degree_7():
+    t, p, w = nodes()
+    x = np.sin(t) * np.cos(p)
+    y = np.sin(t) * np.sin(p)
+    z = np.cos(t)
+    for a in range(8):
+        for b in range(8 - a):
+            for c in range(8 - a - b):
+                q = float(np.sum(w * x ** a * y ** b * z ** c))
+                assert abs(q - sphere_moment(a, b, c)) < 2e-9, (a, b, c, q)
+    # and it is not exact at degree 8 (a rule of degree 7, not more)
+    q = float(np.sum(w * z ** 8))
+    assert abs(q - sphere_moment(0, 0, 8)) > 1e-3
+
+
+@pytest.mark.parametrize("n", range(6))
+def test_pnm_is_condon_shortley_legendre(n):
+    lib = load()
+    for m in range(n + 1):
+        for th in np.linspace(0.05, math.pi - 0.05, 13):
+            got = lib.orc_Pnm(n, m, th)
+            want = lpmv(m, n, math.cos(th))
+            assert abs(got - want) <= 1e-12 * max(1.0, abs(want)), (n, m, th)
+
+
+def test_nnm_normalisation():
+    lib = load()
+    for n in range(6):
+        for m in range(n + 1):
+            want = math.sqrt((2 * n + 1) / (4 * math.pi) * math.factorial(n - m)
+                             / math.factorial(n + m))
+            assert abs(lib.orc_Nnm(n, m) - want) < 1e-14 * want
+
+
+def test_xyz2rtp_round_trip():
+    lib = load()
+    rng = np.random.default_rng(3)
+    r, t, p = C.c_double(), C.c_double(), C.c_double()
+    for x, y, z in rng.normal(size=(200, 3)):
+        lib.orc_xyz2rtp(x, y, z, C.byref(r), C.byref(t), C.byref(p))
+        assert 0 <= t.value <= math.pi and 0 <= p.value < 2 * math.pi + 1e-15
+        back = r.value * np.array([math.sin(t.value) * math.cos(p.value),
+                                   math.sin(t.value) * math.sin(p.value),
+                                   math.cos(t.value)])
+        assert np.allclose(back, [x, y, z], atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# exact Stokes solutions around a fixed sphere
+# ---------------------------------------------------------------------------
+A_RADIUS = 1.0
+COEFFS = ("pnm_re", "pnm_im", "phinm_re", "phinm_im", "chinm_re", "chinm_im")
+N_GRID = 32
+
+
+def _sphere_case():
+    return sim.Case(n=(N_GRID,) * 3, lo=(-2.0,) * 3, hi=(2.0,) * 3, nu=1.0,
+                    rho_f=1.0, init="quiescent", lamb_relax=1.0,
+                    particles=[sim.Particle(x=0.0, y=0.0, z=0.0, r=A_RADIUS,
+                                            order=4, rs_r=1.2, translating=0,
+                                            rotating=0)])
+
+
+def _coords(d, G, sx, sy, sz):
+    i = np.arange(G.inb)
+    j = np.arange(G.jnb)
+    k = np.arange(G.knb)
+    x = (i - sx) * d.dx + d.xs
+    y = (j - sy) * d.dy + d.ys
+    z = (k - sz) * d.dz + d.zs
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    return X, Y, Z
+
+
+def _run_fit(field):
+    """Fill p, u, v, w with an exact solution, fit the Lamb coefficients and
+    return the oracle state plus the exact fields."""
+    s = sim.Simulation(_sphere_case())
+    d = s.dom
+    X, Y, Z = _coords(d, d.Gcc, 0.5, 0.5, 0.5)
+    s.array("p")[:] = field(X, Y, Z)[1].ravel()
+    for name, G, off, ci in (("u", d.Gfx, (1.0, 0.5, 0.5), 0),
+                             ("v", d.Gfy, (0.5, 1.0, 0.5), 1),
+                             ("w", d.Gfz, (0.5, 0.5, 1.0), 2)):
+        X, Y, Z = _coords(d, G, *off)
+        s.array(name)[:] = field(X, Y, Z)[0][ci].ravel()
+    o = OracleState.from_sim(s)
+    s.close()
+    o.call("build_cages")
+    o.call("Lamb")
+    return o
+
+
+def _check_part_bc(o, rel):
+    exact = {c: o.array(c).copy() for c in ("u", "v", "w")}
+    o.call("part_BC")
+    for c in ("u", "v", "w"):
+        a, b = exact[c], o.array(c)
+        changed = a != b
+        assert changed.sum() > 100
+        err = np.abs(a - b)[changed].max()
+        assert err <= rel * np.abs(a[changed]).max(), (c, err)
+
+
+def test_stokes_uniform_flow():
+    W = 0.01
+    a = A_RADIUS
+
+    def field(X, Y, Z):
+        r = np.sqrt(X * X + Y * Y + Z * Z)
+        ct = Z / r
+        ud = W * ct
+        p = -1.5 * W * ct * a / r / r
+        u = []
+        for e, dirc in ((0.0, X / r), (0.0, Y / r), (W, ct)):
+            u.append(-0.75 * a / r * (e + ud * dirc)
+                     - 0.25 * a ** 3 / r ** 3 * (e - 3.0 * ud * dirc) + e)
+        return np.stack(u), p
+
+    o = _run_fit(field)
+    P = o.parts()[0]
+    drag = 6 * math.pi * 1.0 * a * W
+    assert abs(P.Fz / drag - 1.0) < 0.01
+    assert abs(P.Fx) < 1e-3 * drag and abs(P.Fy) < 1e-3 * drag
+    # the disturbance is the decaying n = 1 harmonic: phi_10 carries it
+    # (its pressure is phi's -(3/2)(a/r)^2 term); every other coefficient
+    # is interpolation noise
+    dom = abs(o.array("phinm_re")[1])
+    assert dom > 0
+    for name in COEFFS:
+        c = np.abs(o.array(name))
+        if name == "phinm_re":
+            c = np.delete(c, 1)
+        assert c.max() < 0.01 * dom, (name, c.max() / dom)
+    _check_part_bc(o, 0.02)
+
+
+def test_stokes_straining_flow():
+    G0 = 0.01
+    a = A_RADIUS
+    E = G0 * np.array([[1.0, 0, 0], [0, -1.0, 0], [0, 0, 0]])
+
+    def field(X, Y, Z):
+        r = np.sqrt(X * X + Y * Y + Z * Z)
+        xs = np.stack([X, Y, Z])
+        Ex = np.einsum("ij,j...->i...", E, xs)
+        xEx = np.einsum("i...,i...->...", xs, Ex)
+        u = Ex * (1 - a ** 5 / r ** 5) \
+            - 2.5 * xs * xEx / r ** 2 * (a ** 3 / r ** 3 - a ** 5 / r ** 5)
+        p = -5.0 * a ** 3 * xEx / r ** 5
+        return u, p
+
+    o = _run_fit(field)
+    P = o.parts()[0]
+    scale = 6 * math.pi * a * G0 * a
+    for f in ("Fx", "Fy", "Fz", "Lx", "Ly", "Lz"):
+        assert abs(getattr(P, f)) < 1e-3 * scale, f
+    # x^2 - y^2 is the m = 2 harmonic of degree 2 (index 3 + 2 = 5); its
+    # decaying pressure comes through phi_22, so p_nm stays small
+    dom = abs(o.array("phinm_re")[5])
+    assert dom > 0
+    for name in COEFFS:
+        c = np.abs(o.array(name))
+        if name == "phinm_re":
+            c = np.delete(c, 5)
+        lim = 0.05 if name.startswith("pnm") else 0.01
+        assert c.max() < lim * dom, (name, c.max() / dom)
+    _check_part_bc(o, 0.02)
