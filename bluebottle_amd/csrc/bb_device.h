@@ -11,6 +11,7 @@
 #include <stdint.h>
 #include "../../include/bluebottle.h"
 
+#define BB_EXIT_HIP 86
 void bb_fatal(const char *what, hipError_t err, const char *file, int line);
 
 #define BB_CHECK(call)                                                      \
@@ -76,7 +77,7 @@ struct BBContext {
   int *face_list;           /* flagged faces, packed per component */
   int face_cap;
   int *face_count;          /* device, 3 counters */
-  int face_count_host[3];
+  int *face_count_host;      /* pinned host copy of the 3 counters */
   int *bin_ind, *bin_bin, *bin_start, *bin_end;
   int bin_cap, nbins_cap;
   double *quad;             /* nparts x 26 x 4 node values */
@@ -96,6 +97,10 @@ void bb_free_workspace(void);
 /* Particle workspace sized for nparts and the current grid. */
 void bb_ensure_particle_workspace(void);
 void bb_invalidate_face_lists(void);
+/* copies between the driver's (pageable) host arrays and device memory,
+ * through the library's pinned staging buffer; both return when done */
+void bb_h2d(void *dev, const void *host, size_t bytes);
+void bb_d2h(void *host, const void *dev, size_t bytes);
 
 /* ---- kernel timing (enabled by bb_prof_enable) ---- */
 #define BB_PROF_MAX_KINDS 32

@@ -17,12 +17,14 @@
 
 BBContext g_bb;
 
+/* A failed HIP call ends the process with its own status (BB_EXIT_HIP), so
+ * that a caller can tell a device failure from an ordinary error exit. */
 void bb_fatal(const char *what, hipError_t err, const char *file, int line)
 {
   fprintf(stderr, "bluebottle_hip: %s failed at %s:%d: %s\n", what, file, line,
     hipGetErrorString(err));
   fflush(stderr);
-  exit(EXIT_FAILURE);
+  exit(BB_EXIT_HIP);
 }
 
 BBGeom bb_geom(void)
@@ -70,6 +72,74 @@ static void *dmalloc(size_t bytes)
 static void dfree(void *ptr)
 {
   if(ptr) BB_CHECK(hipFree(ptr));
+}
+
+/* ---------------------------------------------------------------------
+ * Host <-> device transfers of the driver's arrays.  Those arrays are
+ * ordinary (pageable) heap memory owned by the driver; every copy goes
+ * through one pinned staging buffer that this library allocates with
+ * hipHostMalloc, so the runtime never has to register the driver's pages
+ * for DMA.  Chunks alternate between the two halves of the buffer so that
+ * the host-side memcpy of one chunk overlaps the DMA of the other.
+ * ------------------------------------------------------------------- */
+#define BB_STAGE_HALF ((size_t) 16 << 20)
+static char *g_stage = NULL;
+static hipEvent_t g_stage_ev[2];
+
+static void ensure_stage(void)
+{
+  if(g_stage) return;
+  BB_CHECK(hipHostMalloc((void **) &g_stage, 2 * BB_STAGE_HALF,
+    hipHostMallocDefault));
+  BB_CHECK(hipEventCreate(&g_stage_ev[0]));
+  BB_CHECK(hipEventCreate(&g_stage_ev[1]));
+}
+
+void bb_h2d(void *dev, const void *host, size_t bytes)
+{
+  bb_ensure_context();
+  ensure_stage();
+  const char *h = (const char *) host;
+  char *d = (char *) dev;
+  int half = 0;
+  int used[2] = {0, 0};
+  for(size_t off = 0; off < bytes; off += BB_STAGE_HALF, half ^= 1) {
+    size_t n = bytes - off < BB_STAGE_HALF ? bytes - off : BB_STAGE_HALF;
+    char *buf = g_stage + half * BB_STAGE_HALF;
+    /* the DMA that last read this half must be finished before refilling */
+    if(used[half]) BB_CHECK(hipEventSynchronize(g_stage_ev[half]));
+    memcpy(buf, h + off, n);
+    BB_CHECK(hipMemcpyAsync(d + off, buf, n, hipMemcpyHostToDevice, g_bb.stream));
+    BB_CHECK(hipEventRecord(g_stage_ev[half], g_bb.stream));
+    used[half] = 1;
+  }
+  BB_CHECK(hipStreamSynchronize(g_bb.stream));
+}
+
+void bb_d2h(void *host, const void *dev, size_t bytes)
+{
+  bb_ensure_context();
+  ensure_stage();
+  char *h = (char *) host;
+  const char *d = (const char *) dev;
+  size_t nchunk = (bytes + BB_STAGE_HALF - 1) / BB_STAGE_HALF;
+  /* queue chunk q+1 before draining chunk q */
+  for(size_t q = 0; q <= nchunk; q++) {
+    if(q < nchunk) {
+      size_t off = q * BB_STAGE_HALF;
+      size_t n = bytes - off < BB_STAGE_HALF ? bytes - off : BB_STAGE_HALF;
+      char *buf = g_stage + (q & 1) * BB_STAGE_HALF;
+      BB_CHECK(hipMemcpyAsync(buf, d + off, n, hipMemcpyDeviceToHost, g_bb.stream));
+      BB_CHECK(hipEventRecord(g_stage_ev[q & 1], g_bb.stream));
+    }
+    if(q > 0) {
+      size_t p = q - 1;
+      size_t off = p * BB_STAGE_HALF;
+      size_t n = bytes - off < BB_STAGE_HALF ? bytes - off : BB_STAGE_HALF;
+      BB_CHECK(hipEventSynchronize(g_stage_ev[p & 1]));
+      memcpy(h + off, g_stage + (p & 1) * BB_STAGE_HALF, n);
+    }
+  }
 }
 
 void bb_ensure_workspace(void)
@@ -176,14 +246,11 @@ extern "C" void cuda_dom_malloc(void)
 extern "C" void cuda_dom_push(void)
 {
   bb_ensure_context();
-  BB_CHECK(hipMemcpy(_dom[0], &dom[0], sizeof(dom_struct), hipMemcpyHostToDevice));
+  bb_h2d(_dom[0], &dom[0], sizeof(dom_struct));
   for(auto &f : dom_fields()) {
     if(*f.host == NULL) continue;
-    BB_CHECK(hipMemcpyAsync((*f.dev)[0], *f.host,
-      (size_t) grid_count(f.grid) * sizeof(real), hipMemcpyHostToDevice,
-      g_bb.stream));
+    bb_h2d((*f.dev)[0], *f.host, (size_t) grid_count(f.grid) * sizeof(real));
   }
-  BB_CHECK(hipStreamSynchronize(g_bb.stream));
 }
 
 extern "C" void cuda_dom_pull(void)
@@ -191,11 +258,8 @@ extern "C" void cuda_dom_pull(void)
   bb_ensure_context();
   for(auto &f : dom_fields()) {
     if(*f.host == NULL) continue;
-    BB_CHECK(hipMemcpyAsync(*f.host, (*f.dev)[0],
-      (size_t) grid_count(f.grid) * sizeof(real), hipMemcpyDeviceToHost,
-      g_bb.stream));
+    bb_d2h(*f.host, (*f.dev)[0], (size_t) grid_count(f.grid) * sizeof(real));
   }
-  BB_CHECK(hipStreamSynchronize(g_bb.stream));
 }
 
 extern "C" void cuda_dom_free(void)
@@ -270,21 +334,14 @@ extern "C" void cuda_part_push(void)
 {
   bb_ensure_context();
   if(nparts > 0)
-    BB_CHECK(hipMemcpyAsync(_parts[0], parts, (size_t) nparts * sizeof(part_struct),
-      hipMemcpyHostToDevice, g_bb.stream));
+    bb_h2d(_parts[0], parts, (size_t) nparts * sizeof(part_struct));
   if(coeff_bytes() > 0)
     for(auto &c : coeff_fields())
-      if(*c.host)
-        BB_CHECK(hipMemcpyAsync((*c.dev)[0], *c.host, coeff_bytes(),
-          hipMemcpyHostToDevice, g_bb.stream));
+      if(*c.host) bb_h2d((*c.dev)[0], *c.host, coeff_bytes());
   for(auto &f : int_fields())
     if(*f.host)
-      BB_CHECK(hipMemcpyAsync((*f.dev)[0], *f.host,
-        (size_t) grid_count(f.grid) * sizeof(int), hipMemcpyHostToDevice,
-        g_bb.stream));
-  BB_CHECK(hipMemcpyAsync(_binDom, &binDom, sizeof(dom_struct),
-    hipMemcpyHostToDevice, g_bb.stream));
-  BB_CHECK(hipStreamSynchronize(g_bb.stream));
+      bb_h2d((*f.dev)[0], *f.host, (size_t) grid_count(f.grid) * sizeof(int));
+  bb_h2d(_binDom, &binDom, sizeof(dom_struct));
   bb_invalidate_face_lists();
 }
 
@@ -292,19 +349,13 @@ extern "C" void cuda_part_pull(void)
 {
   bb_ensure_context();
   if(nparts > 0)
-    BB_CHECK(hipMemcpyAsync(parts, _parts[0], (size_t) nparts * sizeof(part_struct),
-      hipMemcpyDeviceToHost, g_bb.stream));
+    bb_d2h(parts, _parts[0], (size_t) nparts * sizeof(part_struct));
   if(coeff_bytes() > 0)
     for(auto &c : coeff_fields())
-      if(*c.host)
-        BB_CHECK(hipMemcpyAsync(*c.host, (*c.dev)[0], coeff_bytes(),
-          hipMemcpyDeviceToHost, g_bb.stream));
+      if(*c.host) bb_d2h(*c.host, (*c.dev)[0], coeff_bytes());
   for(auto &f : int_fields())
     if(*f.host)
-      BB_CHECK(hipMemcpyAsync(*f.host, (*f.dev)[0],
-        (size_t) grid_count(f.grid) * sizeof(int), hipMemcpyDeviceToHost,
-        g_bb.stream));
-  BB_CHECK(hipStreamSynchronize(g_bb.stream));
+      bb_d2h(*f.host, (*f.dev)[0], (size_t) grid_count(f.grid) * sizeof(int));
 }
 
 extern "C" void cuda_part_free(void)
@@ -322,6 +373,8 @@ extern "C" void cuda_part_free(void)
   dfree(g_bb.bin_end); dfree(g_bb.quad); dfree(g_bb.coeff_err);
   dfree(g_bb.node_t); dfree(g_bb.node_p);
   if(g_bb.host_err) BB_CHECK(hipHostFree(g_bb.host_err));
+  if(g_bb.face_count_host) BB_CHECK(hipHostFree(g_bb.face_count_host));
+  g_bb.face_count_host = NULL;
   g_bb.face_list = NULL; g_bb.face_count = NULL; g_bb.face_cap = 0;
   g_bb.bin_ind = g_bb.bin_bin = g_bb.bin_start = g_bb.bin_end = NULL;
   g_bb.bin_cap = g_bb.nbins_cap = 0;

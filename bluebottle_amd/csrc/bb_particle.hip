@@ -58,6 +58,9 @@ void bb_ensure_particle_workspace(void)
     g_bb.face_cap = 3 * cap;
   }
   if(!g_bb.face_count) BB_CHECK(hipMalloc(&g_bb.face_count, 4 * sizeof(int)));
+  if(!g_bb.face_count_host)
+    BB_CHECK(hipHostMalloc((void **) &g_bb.face_count_host, 4 * sizeof(int),
+      hipHostMallocDefault));
   int nb = binDom.Gcc.s3 > 0 ? binDom.Gcc.s3 : 1;
   int np = nparts > 0 ? nparts : 1;
   if(g_bb.bin_cap < np || g_bb.nbins_cap < nb) {

@@ -10,8 +10,8 @@
  * that a test can copy the library's state in, run one entry point on both
  * sides, and compare array by array.
  *
- * Parity status per function (pins are listed in DESIGN.md and exercised in
- * tests/test_oracle_*.py): see the comment above each prototype.
+ * Parity status per function: see the comment above each prototype; the
+ * pins are exercised in tests/test_oracle_*.py.
  */
 #ifndef BB_ORACLE_H
 #define BB_ORACLE_H

@@ -7,6 +7,8 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <sys/mman.h>
+
 #include <barrier>
 #include <condition_variable>
 #include <map>
@@ -204,13 +206,80 @@ hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned)
 hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 hipError_t hipDeviceSynchronize(void) { return hipSuccess; }
 
+/* BB_EMU_GUARD=right (or left) places every device allocation against a
+ * 1 GiB band of inaccessible address space on both sides, its end (or its
+ * start) flush with a page boundary, so that any access past it -- however
+ * far -- faults at once instead of landing in another heap block. */
+static int guard_mode(void)
+{
+  static int m = -1;
+  if(m < 0) {
+    const char *e = getenv("BB_EMU_GUARD");
+    m = !e ? 0 : strcmp(e, "right") == 0 ? 1 : strcmp(e, "left") == 0 ? 2 : 0;
+  }
+  return m;
+}
+
+static const size_t GUARD = (size_t) 1 << 30;
+static std::map<uintptr_t, std::pair<uintptr_t, size_t>> g_maps;  /* ptr -> (map, len) */
+
+static void *guarded_alloc(size_t n)
+{
+  size_t pg = 4096;
+  size_t body = (n + pg - 1) / pg * pg;
+  if(body == 0) body = pg;
+  size_t len = body + 2 * GUARD;
+  void *m = mmap(NULL, len, PROT_NONE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE,
+    -1, 0);
+  if(m == MAP_FAILED) return NULL;
+  char *b = (char *) m + GUARD;
+  if(mprotect(b, body, PROT_READ | PROT_WRITE) != 0) return NULL;
+  char *ptr = b;
+  if(guard_mode() == 1) {
+    size_t r = (n + 15) / 16 * 16;
+    ptr = b + body - r;
+  }
+  std::lock_guard<std::mutex> l(g_reg_mu);
+  g_maps[(uintptr_t) ptr] = std::make_pair((uintptr_t) m, len);
+  return ptr;
+}
+
+static bool guarded_free(void *p)
+{
+  std::pair<uintptr_t, size_t> mp;
+  {
+    std::lock_guard<std::mutex> l(g_reg_mu);
+    auto it = g_maps.find((uintptr_t) p);
+    if(it == g_maps.end()) return false;
+    mp = it->second;
+    g_maps.erase(it);
+  }
+  munmap((void *) mp.first, mp.second);
+  return true;
+}
+
 hipError_t hipMalloc(void **p, size_t n)
 {
-  *p = malloc(n ? n : 1);
+  *p = guard_mode() ? guarded_alloc(n) : malloc(n ? n : 1);
   if(!*p) return hipErrorOutOfMemory;
-  /* fill with a signalling pattern: reads of never-written device memory
-   * show up as NaN instead of zero */
-  memset(*p, 0xff, n);
+  /* fill never-written device memory with a pattern: 0xff bytes by
+   * default (NaN doubles, -1 ints); BB_EMU_FILL=zero or =random tries the
+   * other contents a fresh GPU allocation can have */
+  static int mode = -1;
+  if(mode < 0) {
+    const char *e = getenv("BB_EMU_FILL");
+    mode = !e ? 0 : strcmp(e, "zero") == 0 ? 1 : strcmp(e, "random") == 0 ? 2 : 0;
+  }
+  if(mode == 0) memset(*p, 0xff, n);
+  else if(mode == 1) memset(*p, 0, n);
+  else {
+    static uint64_t st = 0x9E3779B97F4A7C15ull;
+    unsigned char *b = (unsigned char *) *p;
+    for(size_t q = 0; q < n; q++) {
+      st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+      b[q] = (unsigned char) st;
+    }
+  }
   reg_add(*p, n);
   return hipSuccess;
 }
@@ -221,7 +290,7 @@ hipError_t hipFree(void *p)
     fprintf(stderr, "bb_emu: hipFree of %p, not a device allocation\n", p);
     abort();
   }
-  free(p);
+  if(!guarded_free(p)) free(p);
   return hipSuccess;
 }
 hipError_t hipHostMalloc(void **p, size_t n, unsigned)
@@ -268,6 +337,7 @@ hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t)
 struct bb_emu_event { int dummy; };
 hipError_t hipEventCreate(hipEvent_t *e) { *e = new bb_emu_event(); return hipSuccess; }
 hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0.f; return hipSuccess; }
 hipError_t hipGetLastError(void) { return hipSuccess; }
 const char *hipGetErrorString(hipError_t) { return "emulated error"; }

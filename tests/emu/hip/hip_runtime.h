@@ -86,6 +86,7 @@ hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t st = 0);
 hipError_t hipEventCreate(hipEvent_t *e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s = 0);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
+hipError_t hipEventSynchronize(hipEvent_t e);
 hipError_t hipGetLastError(void);
 const char *hipGetErrorString(hipError_t e);
 

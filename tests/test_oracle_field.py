@@ -434,3 +434,48 @@ def test_solvability_zeroes_net_flux():
     fy = (vs[1:-1, d.Gfy.je - 1, 1:-1] - vs[1:-1, d.Gfy.js, 1:-1]).sum() * d.dx * d.dz
     fz = (ws[d.Gfz.ke - 1, 1:-1, 1:-1] - ws[d.Gfz.ks, 1:-1, 1:-1]).sum() * d.dx * d.dy
     assert abs(fx + fy + fz) < 1e-11
+
+
+def test_forcing_ramps():
+    """f = -gradP / rho_f with gradP ramping at rate `a` from the delay
+    until it reaches `m` (a = 0: m at once); before the delay gradP keeps
+    its value."""
+    case = configs.periodic_case(8, 0.01)
+    o, d = state(case)
+    o.s.rho_f = 2.0
+    o.s.p_tDelay = 0.1
+    gp = o.s.gradP
+    gp.x, gp.y, gp.z = 9.0, 9.0, 9.0
+    gp.xm, gp.xa = 2.0, 0.5
+    gp.ym, gp.ya = -3.0, 0.0
+    gp.zm, gp.za = -1.0, -4.0
+    for t, want in ((0.05, (9.0, 9.0, 9.0)), (1.1, (0.5, -3.0, -1.0)),
+                    (0.2, (0.05, -3.0, -0.4)), (20.0, (2.0, -3.0, -1.0))):
+        gp.x, gp.y, gp.z = 9.0, 9.0, 9.0
+        o.s.ttime = t
+        o.call("compute_forcing")
+        got = (o.s.gradP.x, o.s.gradP.y, o.s.gradP.z)
+        assert np.allclose(got, want, rtol=1e-14, atol=1e-15), (t, got)
+        for c, w in (("f_x", want[0]), ("f_y", want[1]), ("f_z", want[2])):
+            assert np.allclose(o.array(c), -w / 2.0, rtol=1e-14, atol=0)
+
+
+def test_history_shift_copies():
+    case = configs.small_particle_case(n=8, nparts=1, radius_dx=2.0)
+    o, d = state(case)
+    rng = np.random.default_rng(7)
+    for name in ("conv_u", "conv_v", "conv_w", "diff_u", "diff_v", "diff_w",
+                 "u", "v", "w", "p"):
+        o.array(name)[:] = rng.normal(size=o.array(name).size)
+    o.call("store_u")
+    for a, b in (("conv_u", "conv0_u"), ("conv_v", "conv0_v"),
+                 ("conv_w", "conv0_w"), ("diff_u", "diff0_u"),
+                 ("diff_v", "diff0_v"), ("diff_w", "diff0_w"), ("u", "u0"),
+                 ("v", "v0"), ("w", "w0"), ("p", "p0")):
+        assert np.array_equal(o.array(a), o.array(b)), b
+    from oracle.pyoracle import COEFF_NAMES
+    for nm in COEFF_NAMES[:6]:
+        o.array(nm)[:] = rng.normal(size=o.array(nm).size)
+    o.call("store_coeffs")
+    for nm, nm00 in zip(COEFF_NAMES[:6], COEFF_NAMES[12:18]):
+        assert np.array_equal(o.array(nm), o.array(nm00))

@@ -249,3 +249,189 @@ def test_lamb_err_brute_force():
     assert got == want
     for nm, nm0 in zip(COEFF_NAMES[:6], COEFF_NAMES[6:12]):
         assert np.array_equal(o.array(nm0), new[nm])
+
+
+# ---------------------------------------------------------------------------
+# pair forces: lubrication and contact
+# ---------------------------------------------------------------------------
+def _pair(gap, U, a=0.1, E=1.0e6, sigma=0.5):
+    """Two equal spheres on the x axis, gap `gap` between surfaces; the
+    first moves with +U towards the second (neither is integrated, so both
+    collision passes see the same velocities)."""
+    case = configs.small_particle_case(n=16, nparts=2, radius_dx=1.0,
+                                       taylor_green=False)
+    x0 = 0.3
+    for P, x in zip(case.particles, (x0, x0 + 2 * a + gap)):
+        P.x, P.y, P.z, P.r = x, 0.5, 0.5, a
+        P.translating, P.rotating = 0, 0
+        P.E, P.sigma = E, sigma
+    o, d = _state(case)
+    o.s.dt = 1e-4
+    o.s.parts[0].u = U
+    o.s.parts[0].u0 = U
+    o.call("move_parts_sub")
+    return o, o.parts()
+
+
+def test_lubrication_reynolds_limit_and_third_law():
+    a, U = 0.1, 0.05
+    o, (Pi, Pj) = _pair(0.02 * a, U, a)
+    mu = o.s.mu
+    reynolds = -1.5 * math.pi * mu * a * a * U / (0.02 * a)
+    # the squeeze force opposes the approach and tends to Reynolds' law
+    assert Pi.iFx < 0 < Pj.iFx
+    assert abs(Pi.iFx / reynolds - 1) < 0.1
+    assert abs(Pi.iFx + Pj.iFx) < 1e-12 * abs(Pi.iFx)
+    for f in ("iFy", "iFz", "iLx", "iLy", "iLz"):
+        assert abs(getattr(Pi, f)) < 1e-12 * abs(Pi.iFx)
+    # closer gaps push harder, and the force is linear in the velocity
+    o2, (Qi, _) = _pair(0.01 * a * 2, 2 * U, a)
+    assert abs(Qi.iFx / Pi.iFx - 2.0) < 1e-12
+
+
+def test_no_pair_force_beyond_interaction_length():
+    a = 0.1
+    o, (Pi, Pj) = _pair(1.05 * a, 0.05, a)   # hN = r (ratio 1)
+    for P in (Pi, Pj):
+        for f in ("iFx", "iFy", "iFz", "iLx", "iLy", "iLz"):
+            assert getattr(P, f) == 0.0
+
+
+def test_hertz_contact_at_rest():
+    a, E, sigma = 0.1, 2.0e5, 0.3
+    delta = 0.01 * a
+    o, (Pi, Pj) = _pair(-delta, 0.0, a, E=E, sigma=sigma)
+    estar = 1.0 / (2 * (1 - sigma ** 2) / E)
+    rstar = 1.0 / (2.0 / a)
+    hertz = 4.0 / 3.0 * estar * math.sqrt(rstar) * delta ** 1.5
+    # repulsive along the line of centres, equal and opposite
+    assert abs(-Pi.iFx / hertz - 1) < 1e-12
+    assert abs(Pj.iFx / hertz - 1) < 1e-12
+    assert abs(Pi.iFy) < 1e-12 * hertz and abs(Pi.iFz) < 1e-12 * hertz
+
+
+# ---------------------------------------------------------------------------
+# rigid-body fills inside particles
+# ---------------------------------------------------------------------------
+def _rigid_case():
+    case = _one_sphere()
+    P = case.particles[0]
+    P.x, P.y, P.z = 0.47, 0.52, 0.5
+    return case
+
+
+def _min_image(v, L):
+    return v - L * np.round(v / L)
+
+
+def test_parts_internal_is_rigid_body_motion():
+    o, d = _state(_rigid_case())
+    o.call("build_cages")
+    P = o.s.parts[0]
+    U = np.array([0.1, -0.2, 0.05])
+    Om = np.array([0.7, 0.3, -1.2])
+    P.u, P.v, P.w = U
+    P.ox, P.oy, P.oz = Om
+    rng = np.random.default_rng(1)
+    before = {}
+    for c, G in (("u", d.Gfx), ("v", d.Gfy), ("w", d.Gfz)):
+        o.array(c)[:] = rng.normal(size=G.s3b)
+        before[c] = o.array(c).copy()
+    o.call("parts_internal")
+    Gc = d.Gcc
+    ph = o.array("phase").reshape(Gc.knb, Gc.jnb, Gc.inb)
+    changed = 0
+    for ax, (c, G, off) in enumerate((("u", d.Gfx, (1.0, 0.5, 0.5)),
+                                      ("v", d.Gfy, (0.5, 1.0, 0.5)),
+                                      ("w", d.Gfz, (0.5, 0.5, 1.0)))):
+        fl = o.array("flag_" + c).reshape(G.knb, G.jnb, G.inb)
+        got = o.array(c).reshape(G.knb, G.jnb, G.inb)
+        old = before[c].reshape(G.knb, G.jnb, G.inb)
+        for k in range(1, G.knb - 1):
+            for j in range(1, G.jnb - 1):
+                for i in range(1, G.inb - 1):
+                    cell = [k, j, i]
+                    lo = list(cell)
+                    lo[2 - ax] -= 1
+                    if not (0 <= lo[2 - ax]) or cell[2 - ax] > (Gc.knb, Gc.jnb, Gc.inb)[2 - ax] - 2:
+                        continue
+                    inside = ph[tuple(lo)] > -1 and ph[tuple(cell)] > -1 \
+                        and fl[k, j, i] != -1
+                    if not inside:
+                        assert got[k, j, i] == old[k, j, i]
+                        continue
+                    pos = np.array([d.xs + (i - off[0]) * d.dx,
+                                    d.ys + (j - off[1]) * d.dy,
+                                    d.zs + (k - off[2]) * d.dz])
+                    r = _min_image(pos - np.array([P.x, P.y, P.z]), d.xl)
+                    want = (U + np.cross(Om, r))[ax]
+                    assert abs(got[k, j, i] - want) < 1e-14
+                    changed += 1
+    assert changed > 20
+
+
+def test_p_fill_is_rigid_body_pressure():
+    """Inside a particle the pressure is the n = 0 Lamb term plus the
+    pressure of a rigid body's frame: centrifugal 1/2 rho |Omega x r|^2 and
+    the hydrostatic (-gradP - rho a_P) . r."""
+    o, d = _state(_rigid_case())
+    o.call("build_cages")
+    P = o.s.parts[0]
+    Om = np.array([0.4, -0.8, 1.1])
+    acc = np.array([0.3, -0.1, -0.6])
+    P.ox, P.oy, P.oz = Om
+    P.udot, P.vdot, P.wdot = acc
+    gp = np.array([0.05, 0.2, -0.7])
+    o.s.gradP.x, o.s.gradP.y, o.s.gradP.z = gp
+    c00 = 0.37
+    o.array("pnm_re")[:] = 0.0
+    o.array("pnm_re")[0] = c00
+    rng = np.random.default_rng(2)
+    Gc = d.Gcc
+    o.array("p")[:] = rng.normal(size=Gc.s3b)
+    old = o.array("p").copy().reshape(Gc.knb, Gc.jnb, Gc.inb)
+    o.call("part_p_fill")
+    got = o.array("p").reshape(Gc.knb, Gc.jnb, Gc.inb)
+    ph = o.array("phase").reshape(Gc.knb, Gc.jnb, Gc.inb)
+    rho, mu, nu, a = o.s.rho_f, o.s.mu, o.s.nu, P.r
+    n00 = 1.0 / (2.0 * math.sqrt(math.pi))
+    nsolid = 0
+    for k in range(1, Gc.knb - 1):
+        for j in range(1, Gc.jnb - 1):
+            for i in range(1, Gc.inb - 1):
+                if ph[k, j, i] < 0:
+                    assert got[k, j, i] == old[k, j, i]
+                    continue
+                nsolid += 1
+                pos = np.array([d.xs + (i - 0.5) * d.dx, d.ys + (j - 0.5) * d.dy,
+                                d.zs + (k - 0.5) * d.dz])
+                r = _min_image(pos - np.array([P.x, P.y, P.z]), d.xl)
+                want = c00 * n00 * mu * nu / a ** 2 \
+                    + 0.5 * rho * np.dot(np.cross(Om, r), np.cross(Om, r)) \
+                    + np.dot(-gp - rho * acc, r)
+                assert abs(got[k, j, i] - want) < 1e-13
+    assert nsolid > 50
+
+
+def test_part_bc_p_masks_rhs():
+    """The particle pass on the Poisson right-hand side keeps it on fluid
+    cells away from particles and zeroes it on solid cells and on the
+    fluid side of the solid shell."""
+    case = configs.small_particle_case(n=16, nparts=2, radius_dx=3.0)
+    o, d = _state(case)
+    o.call("build_cages")
+    rng = np.random.default_rng(3)
+    for c, G in (("u_star", d.Gfx), ("v_star", d.Gfy), ("w_star", d.Gfz)):
+        o.array(c)[:] = rng.normal(size=G.s3b)
+    o.s.dt = 1e-3
+    o.call("PP_rhs")
+    rhs = o.array("rhs_p").copy()
+    o.call("part_BC_p")
+    Gc = d.Gcc
+    ph = o.array("phase").reshape(Gc.knb, Gc.jnb, Gc.inb)[1:-1, 1:-1, 1:-1].ravel()
+    sh = o.array("phase_shell").reshape(Gc.knb, Gc.jnb, Gc.inb)[1:-1, 1:-1, 1:-1].ravel()
+    keep = (ph < 0) & (sh > 0)
+    got = o.array("rhs_p")
+    assert np.array_equal(got[keep], rhs[keep])
+    assert np.all(got[~keep] == 0.0)
+    assert (~keep).sum() > 50
