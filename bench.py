@@ -141,7 +141,7 @@ def roofline(prof, nbytes):
             "bytes_per_launch": nbytes[name]}, table
 
 
-def cpu_baseline(s, fp64, krylov_sample):
+def cpu_baseline(s, fp64, krylov_sample, desc):
     """Serial FP64 oracle on this host, bounded sample, extrapolated."""
     import ctypes as C
     from oracle.pyoracle import OracleState
@@ -214,14 +214,14 @@ def cpu_baseline(s, fp64, krylov_sample):
     return {
         "value": round(1.0 / step_s, 6), "unit": "timesteps/s", "cores": 1,
         "kind": "port",
-        "sample": ("serial FP64 oracle on the full 256^3 + 1000-sphere state: "
+        "sample": ("serial FP64 oracle on the full state of the workload (%s): "
                    "one pass of each sub-iteration stage (%.1f s), the step's "
                    "outer stages (%.1f s) and %d BiCGSTAB iterations "
                    "(%.2f s/iteration), %.1f s measured in all; extrapolated "
                    "to %.2f Lamb sub-iterations x %.1f Krylov iterations per "
                    "step as the GPU FP64 run measured" % (
-                       stage, outer, krylov_sample, per_iter, sample_s, li,
-                       ki)),
+                       desc, stage, outer, krylov_sample, per_iter, sample_s,
+                       li, ki)),
     }
 
 
@@ -232,6 +232,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--nparts", type=int, default=1000)
+    ap.add_argument("--workload", choices=("auto", "sediment", "tg"),
+                    default="auto",
+                    help="sediment: 256^3 with --nparts spheres (needs the "
+                         "particle kernels, BB_PARTICLE_KERNELS=1); tg: the "
+                         "same box as a sphere-free Taylor-Green flow; auto: "
+                         "sediment when the particle kernels are enabled")
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--krylov-sample", type=int, default=10)
@@ -243,8 +249,21 @@ def main():
         sys.exit(2)
 
     from bluebottle_amd import configs, runtime, sim
-    case = configs.config3(n=args.n, nparts=args.nparts)
-    log("setting up %d^3 with %d spheres" % (args.n, args.nparts))
+    workload = args.workload
+    if workload == "auto":
+        workload = ("sediment" if os.environ.get("BB_PARTICLE_KERNELS") == "1"
+                    else "tg")
+    if workload == "sediment":
+        case = configs.config3(n=args.n, nparts=args.nparts)
+        desc = ("sedimentation %dx%dx%d periodic, %d spheres r=6dx, "
+                "rho_p/rho_f=2, Re_p=10, L=4, 26 nodes, Poisson rel tol 1e-8"
+                % (args.n, args.n, args.n, args.nparts))
+    else:
+        case = configs.taylor_green(n=args.n, pp_residual=1e-8)
+        desc = ("Taylor-Green %dx%dx%d periodic unit box, Re=100, no spheres "
+                "(particle kernels off on the GPU), Poisson rel tol 1e-8"
+                % (args.n, args.n, args.n))
+    log("setting up %s" % desc)
     t0 = time.perf_counter()
     s = sim.Simulation(case)
     s.to_device()
@@ -268,7 +287,7 @@ def main():
     cpu = None
     if not args.no_cpu and "fp64" in modes:
         log("timing the CPU oracle sample")
-        cpu = cpu_baseline(s, modes["fp64"], args.krylov_sample)
+        cpu = cpu_baseline(s, modes["fp64"], args.krylov_sample, desc)
         log("cpu sample done")
     out = {
         "metric": METRIC,
@@ -284,9 +303,7 @@ def main():
         "dtype": "f64+f32" if head == "mixed" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": "sedimentation %dx%dx%d periodic, %d spheres r=6dx, "
-                        "rho_p/rho_f=2, Re_p=10, L=4, 26 nodes, Poisson rel "
-                        "tol 1e-8" % (xn, yn, zn, args.nparts),
+            "workload": desc,
             "solver": head + " BiCGSTAB, Jacobi preconditioner",
             "lamb_max_iter": case.lamb_max_iter,
         },

@@ -77,6 +77,14 @@ def config1(n=48, lamb_max_iter=3, pp_residual=1e-10, nparts=1, order=4):
                 init="taylor_green", noise=1e-3, seed=1, particles=parts)
 
 
+def taylor_green(n=256, pp_residual=1e-8, pp_max_iter=20000, re=100.0):
+    """Sphere-free periodic Taylor-Green box (configuration 1's flow and
+    noise) at n^3, for timing the fluid path alone."""
+    return Case(n=(n, n, n), nu=1.0 / re, CFL=0.3, pp_max_iter=pp_max_iter,
+                pp_residual=pp_residual, lamb_max_iter=1,
+                init="taylor_green", noise=1e-3, seed=1, particles=[])
+
+
 def sedimentation(n, nparts, radius_dx, density_ratio, re_p, seed,
                   pp_residual=1e-8, lamb_max_iter=25, min_gap_dx=1.0,
                   order=4, pp_max_iter=20000):

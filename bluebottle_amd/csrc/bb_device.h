@@ -11,6 +11,18 @@
 #include <stdint.h>
 #include "../../include/bluebottle.h"
 
+/* A driver built as the reference's IMPLICIT variant defines no diff0
+ * history arrays (src/bluebottle.c:60-67): they are referenced weakly and
+ * their absence selects that variant (bb_set_implicit). */
+extern "C" {
+#pragma weak diff0_u
+#pragma weak diff0_v
+#pragma weak diff0_w
+#pragma weak _diff0_u
+#pragma weak _diff0_v
+#pragma weak _diff0_w
+}
+
 #define BB_EXIT_HIP 86
 void bb_fatal(const char *what, hipError_t err, const char *file, int line);
 
@@ -73,6 +85,7 @@ struct BBContext {
   int solver_mode;
   bb_solve_stats stats;
   int exit_on_fail;         /* reference behaviour: exit(EXIT_FAILURE) */
+  int implicit;             /* IMPLICIT variant, fixed at cuda_dom_malloc */
   /* particle workspace */
   int *face_list;           /* flagged faces, packed per component */
   int face_cap;
@@ -94,6 +107,12 @@ void bb_ensure_context(void);
 /* Allocate / grow the Krylov workspace for the current Dom. */
 void bb_ensure_workspace(void);
 void bb_free_workspace(void);
+/* Crank-Nicolson solver workspace (bb_helmholtz.hip) */
+void bb_free_helmholtz(void);
+/* Exits with a message when `entry` would launch the particle boundary,
+ * quadrature or motion kernels (nparts > 0) while they are switched off
+ * (bb_enable_particle_kernels, BB_PARTICLE_KERNELS). */
+void bb_require_particle_kernels(const char *entry);
 /* Particle workspace sized for nparts and the current grid. */
 void bb_ensure_particle_workspace(void);
 void bb_invalidate_face_lists(void);

@@ -176,6 +176,12 @@ k_u_star(BBGeom G, double rho_f, double nu, double dt0, double dt,
 extern "C" void cuda_U_star_2(void)
 {
   bb_ensure_context();
+  if(g_bb.implicit) {
+    /* the IMPLICIT build has no explicit predictor and no diff0 history */
+    fprintf(stderr, "bluebottle_hip: cuda_U_star_2 called in the IMPLICIT "
+      "variant; use cuda_{u,v,w}star_helmholtz\n");
+    exit(EXIT_FAILURE);
+  }
   BBGeom G = bb_geom();
   dim3 blk(TX, TY, 1);
   dim3 grd((G.xn + 1 + TX - 1) / TX, (G.yn + 1 + TY - 1) / TY, G.zn + 1);
@@ -675,12 +681,21 @@ extern "C" real cuda_find_dt(void)
   double u_max = g_bb.ws.red_host[0], v_max = g_bb.ws.red_host[1];
   double w_max = g_bb.ws.red_host[2];
   const dom_struct *d = &dom[0];
-  real tmp = u_max / d->dx + 2.*nu/d->dx/d->dx;
-  tmp += v_max / d->dy + 2.*nu/d->dy/d->dy;
-  tmp += w_max / d->dz + 2.*nu/d->dz/d->dz;
+  real tmp;
+  if(!g_bb.implicit) {
+    tmp = u_max / d->dx + 2.*nu/d->dx/d->dx;
+    tmp += v_max / d->dy + 2.*nu/d->dy/d->dy;
+    tmp += w_max / d->dz + 2.*nu/d->dz/d->dz;
+  } else {
+    /* Crank-Nicolson diffusion: advective limit only, growth capped */
+    tmp = u_max / d->dx;
+    tmp += v_max / d->dy;
+    tmp += w_max / d->dz;
+  }
   real dts = CFL / tmp;
   real mn = FLT_MAX;
   if(dts < mn) mn = dts;
+  if(g_bb.implicit && mn > 1.1*dt) mn = 1.1*dt;
   return mn;
 }
 
@@ -697,9 +712,11 @@ extern "C" void cuda_store_u(void)
   BB_CHECK(hipMemcpyAsync(_conv0_u[0], _conv_u[0], fx, hipMemcpyDeviceToDevice, s));
   BB_CHECK(hipMemcpyAsync(_conv0_v[0], _conv_v[0], fy, hipMemcpyDeviceToDevice, s));
   BB_CHECK(hipMemcpyAsync(_conv0_w[0], _conv_w[0], fz, hipMemcpyDeviceToDevice, s));
-  BB_CHECK(hipMemcpyAsync(_diff0_u[0], _diff_u[0], fx, hipMemcpyDeviceToDevice, s));
-  BB_CHECK(hipMemcpyAsync(_diff0_v[0], _diff_v[0], fy, hipMemcpyDeviceToDevice, s));
-  BB_CHECK(hipMemcpyAsync(_diff0_w[0], _diff_w[0], fz, hipMemcpyDeviceToDevice, s));
+  if(!g_bb.implicit) {
+    BB_CHECK(hipMemcpyAsync(_diff0_u[0], _diff_u[0], fx, hipMemcpyDeviceToDevice, s));
+    BB_CHECK(hipMemcpyAsync(_diff0_v[0], _diff_v[0], fy, hipMemcpyDeviceToDevice, s));
+    BB_CHECK(hipMemcpyAsync(_diff0_w[0], _diff_w[0], fz, hipMemcpyDeviceToDevice, s));
+  }
   BB_CHECK(hipMemcpyAsync(_p0[0], _p[0], cc, hipMemcpyDeviceToDevice, s));
   BB_CHECK(hipMemcpyAsync(_u0[0], _u[0], fx, hipMemcpyDeviceToDevice, s));
   BB_CHECK(hipMemcpyAsync(_v0[0], _v[0], fy, hipMemcpyDeviceToDevice, s));

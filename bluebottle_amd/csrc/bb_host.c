@@ -256,16 +256,20 @@ int bb_host_particles_init(void)
   real rmax = 0;
   for(int i = 0; i < nparts; i++) rmax = rmax + (parts[i].r > rmax)*(parts[i].r - rmax);
   memset(&binDom, 0, sizeof(binDom));
+  /* with no particles rmax = 0 and the reference's floor(xl / 0) to int
+   * is undefined; one bin per axis is used instead (no kernel reads the
+   * bins when nparts = 0) */
+  const real bin_w = rmax > 0 ? 2.*rmax + interactionLengthRatio*rmax : 2.*Dom.xl;
   binDom.xs = Dom.xs; binDom.xe = Dom.xe; binDom.xl = Dom.xl;
-  binDom.xn = floor(Dom.xl/(2.*rmax + interactionLengthRatio*rmax));
+  binDom.xn = rmax > 0 ? floor(Dom.xl/bin_w) : 0;
   if(binDom.xn == 0) { binDom.xn = 1; binDom.dx = Dom.xl; }
   else binDom.dx = Dom.xl / binDom.xn;
   binDom.ys = Dom.ys; binDom.ye = Dom.ye; binDom.yl = Dom.yl;
-  binDom.yn = floor(Dom.yl/(2.*rmax + interactionLengthRatio*rmax));
+  binDom.yn = rmax > 0 ? floor(Dom.yl/bin_w) : 0;
   if(binDom.yn == 0) { binDom.yn = 1; binDom.dy = Dom.yl; }
   else binDom.dy = Dom.yl / binDom.yn;
   binDom.zs = Dom.zs; binDom.ze = Dom.ze; binDom.zl = Dom.zl;
-  binDom.zn = floor(Dom.zl/(2.*rmax + interactionLengthRatio*rmax));
+  binDom.zn = rmax > 0 ? floor(Dom.zl/bin_w) : 0;
   if(binDom.zn == 0) { binDom.zn = 1; binDom.dz = Dom.zl; }
   else binDom.dz = Dom.zl / binDom.zn;
   binDom.Gcc.is = DOM_BUF; binDom.Gcc.in = binDom.xn;
@@ -351,7 +355,12 @@ int bb_host_step(void)
   int iter = 0;
   real iter_err = FLT_MAX;
   while(iter_err > lamb_residual) {
-    cuda_U_star_2();
+    if(!bb_is_implicit()) cuda_U_star_2();
+    else {
+      cuda_ustar_helmholtz(0);
+      cuda_vstar_helmholtz(0);
+      cuda_wstar_helmholtz(0);
+    }
     if(nparts > 0) cuda_part_BC_star();
     cuda_dom_BC_star();
     cuda_solvability();

@@ -215,18 +215,44 @@ static std::vector<FieldDesc> dom_fields(void)
     {&_u, &u, 1}, {&_u0, &u0, 1}, {&_v, &v, 2}, {&_v0, &v0, 2},
     {&_w, &w, 3}, {&_w0, &w0, 3},
     {&_f_x, &f_x, 1}, {&_f_y, &f_y, 2}, {&_f_z, &f_z, 3},
-    {&_diff0_u, &diff0_u, 1}, {&_diff0_v, &diff0_v, 2}, {&_diff0_w, &diff0_w, 3},
     {&_conv0_u, &conv0_u, 1}, {&_conv0_v, &conv0_v, 2}, {&_conv0_w, &conv0_w, 3},
     {&_diff_u, &diff_u, 1}, {&_diff_v, &diff_v, 2}, {&_diff_w, &diff_w, 3},
     {&_conv_u, &conv_u, 1}, {&_conv_v, &conv_v, 2}, {&_conv_w, &conv_w, 3},
     {&_u_star, &u_star, 1}, {&_v_star, &v_star, 2}, {&_w_star, &w_star, 3},
   };
+  /* Adams-Bashforth diffusion history: explicit variant only */
+  if(!g_bb.implicit) {
+    f.push_back({&_diff0_u, &diff0_u, 1});
+    f.push_back({&_diff0_v, &diff0_v, 2});
+    f.push_back({&_diff0_w, &diff0_w, 3});
+  }
   return f;
+}
+
+static int g_implicit_req = -1;
+
+extern "C" void bb_set_implicit(int mode) { g_implicit_req = mode; }
+extern "C" int bb_is_implicit(void) { return g_bb.implicit; }
+
+/* The variant for the domain about to be allocated. */
+static int resolve_implicit(void)
+{
+  const int have_diff0 = &_diff0_u != NULL && &_diff0_v != NULL &&
+    &_diff0_w != NULL && &diff0_u != NULL && &diff0_v != NULL &&
+    &diff0_w != NULL;
+  if(g_implicit_req < 0) return !have_diff0;
+  if(!g_implicit_req && !have_diff0) {
+    fprintf(stderr, "bluebottle_hip: explicit time stepping needs the driver's "
+      "diff0_u/v/w arrays, which it does not define\n");
+    exit(EXIT_FAILURE);
+  }
+  return g_implicit_req != 0;
 }
 
 extern "C" void cuda_dom_malloc(void)
 {
   bb_ensure_context();
+  g_bb.implicit = resolve_implicit();
   _dom = (dom_struct **) malloc(nsubdom * sizeof(dom_struct *));
   _dom[0] = (dom_struct *) dmalloc(sizeof(dom_struct));
   for(auto &f : dom_fields()) {
@@ -272,6 +298,7 @@ extern "C" void cuda_dom_free(void)
   if(_rhs_p) { dfree(_rhs_p[0]); free(_rhs_p); _rhs_p = NULL; }
   if(_dom) { dfree(_dom[0]); free(_dom); _dom = NULL; }
   bb_free_workspace();
+  bb_free_helmholtz();
 }
 
 /* ---------------------------------------------------------------------
@@ -436,7 +463,7 @@ extern "C" void bb_prof_enable(int on)
   bb_ensure_context();
   g_prof_on = on;
   if(on && g_prof_pool.empty()) {
-    g_prof_pool.resize(1 << 15);
+    g_prof_pool.resize(4096);
     for(auto &r : g_prof_pool) {
       BB_CHECK(hipEventCreate(&r.a));
       BB_CHECK(hipEventCreate(&r.b));

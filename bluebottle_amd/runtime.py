@@ -74,6 +74,10 @@ EXTRA_HIP = {
     "bb_prof_reset": (None, []),
     "bb_set_poll_interval": (None, [C.c_int]),
     "bb_set_mixed_params": (None, [C.c_double, C.c_int, C.c_int]),
+    "bb_set_implicit": (None, [C.c_int]),
+    "bb_is_implicit": (C.c_int, []),
+    "bb_enable_particle_kernels": (None, [C.c_int]),
+    "bb_particle_kernels_enabled": (C.c_int, []),
 }
 
 # kernel kinds of the event timer (bb_device.h)

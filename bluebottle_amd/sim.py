@@ -70,6 +70,8 @@ class Case:
     noise: float = 1e-3
     seed: int = 1
     interaction_length_ratio: float = 1.0
+    # Crank-Nicolson diffusion (the reference's -DIMPLICIT build)
+    implicit: bool = False
     particles: list = field(default_factory=list)
 
 
@@ -244,6 +246,7 @@ class Simulation:
     # ------------------------------------------------------------------
     def to_device(self):
         h = self.hip
+        h.bb_set_implicit(1 if self.case.implicit else 0)
         h.cuda_dom_malloc()
         h.cuda_part_malloc()
         h.cuda_dom_push()

@@ -360,6 +360,27 @@ const char *bb_device_name(void);
  * on = 0 keeps the message and returns. */
 void bb_set_exit_on_nonconvergence(int on);
 
+/* Time-integration build the library behaves as.  The reference selects it
+ * at compile time (-DIMPLICIT: Crank-Nicolson diffusion through
+ * cuda_{u,v,w}star_helmholtz, no diff0 history arrays, find_dt without the
+ * viscous limit and capped at 1.1 dt, src/bluebottle.c:60-67,
+ * src/cuda_bluebottle.cu:3526-3574).  mode = 1: IMPLICIT, 0: explicit,
+ * -1 (default): IMPLICIT exactly when the driver defines no diff0 arrays.
+ * Takes effect at the next cuda_dom_malloc. */
+void bb_set_implicit(int mode);
+/* 1 when the allocated domain runs as the IMPLICIT build. */
+int bb_is_implicit(void);
+
+/* The particle boundary-value, internal-velocity, pressure-fill,
+ * quadrature and motion kernels (cuda_part_BC, cuda_part_BC_star,
+ * cuda_part_p_fill, cuda_parts_internal, cuda_Lamb, cuda_lamb_err,
+ * cuda_quad_*, cuda_move_parts*) are off by default: called with
+ * nparts > 0 they print a message and exit.  on = 1, or the environment
+ * variable BB_PARTICLE_KERNELS=1 at the first call, enables them.
+ * Cages, phases and flags (cuda_build_cages) are not affected. */
+void bb_enable_particle_kernels(int on);
+int bb_particle_kernels_enabled(void);
+
 /* Blocks until all work the library queued has finished. */
 void bb_synchronize(void);
 

@@ -59,6 +59,10 @@ typedef struct orc_state {
   int last_iters;
   real last_resid;
   int last_converged;
+  /* 1: the reference's IMPLICIT build (Crank-Nicolson Helmholtz solves
+   * instead of the explicit predictor, no diff0 history, dt without the
+   * viscous limit and capped at 1.1 dt) */
+  int implicit;
 } orc_state;
 
 /* Names of coeff[] entries, in order. */
@@ -146,6 +150,19 @@ void orc_lebedev_nodes(real theta[26], real phi[26], real weight[26]);
 real orc_Nnm(int n, int m);
 real orc_Pnm(int n, int m, real theta);
 void orc_xyz2rtp(real x, real y, real z, real *r, real *theta, real *phi);
+
+/* --- Crank-Nicolson velocity solves (oracle_helmholtz.c) --- */
+/* cuda_{u,v,w}star_helmholtz, src/cuda_bicgstab.cu:44-714 (comp 0, 1, 2). */
+void orc_star_helmholtz(orc_state *s, int comp);
+/* The 13 DIA diagonals of the component's operator, values[C + d*n]
+ * (src/bicgstab_kernel.cu:643-1652 in the driver's order). */
+void orc_helmholtz_coeffs(const orc_state *s, int comp, real *values);
+void orc_helmholtz_matvec(const orc_state *s, int comp, const real *values,
+  const real *x, real *y);
+/* Diagonally preconditioned CG with CUSP's monitor semantics (CUSP is not
+ * vendored: iterate-level parity unpinned, solution pinned by residual). */
+int orc_pcg(const orc_state *s, int comp, const real *values, const real *b,
+  real *x, int max_iter, real rel_tol, real *resid_out, int *converged);
 
 /* --- one full time step (oracle_step.c): src/bluebottle.c:598-711 --- */
 /* Returns the number of Lamb sub-iterations taken. */

@@ -443,12 +443,20 @@ real orc_find_dt(orc_state *s)
   for(int n = 0; n < d->Gfx.s3b; n++) if(s->u[n] > umax) umax = s->u[n];
   for(int n = 0; n < d->Gfy.s3b; n++) if(s->v[n] > vmax) vmax = s->v[n];
   for(int n = 0; n < d->Gfz.s3b; n++) if(s->w[n] > wmax) wmax = s->w[n];
-  real tmp = umax / d->dx + 2.*s->nu/d->dx/d->dx;
-  tmp += vmax / d->dy + 2.*s->nu/d->dy/d->dy;
-  tmp += wmax / d->dz + 2.*s->nu/d->dz/d->dz;
+  real tmp;
+  if(!s->implicit) {
+    tmp = umax / d->dx + 2.*s->nu/d->dx/d->dx;
+    tmp += vmax / d->dy + 2.*s->nu/d->dy/d->dy;
+    tmp += wmax / d->dz + 2.*s->nu/d->dz/d->dz;
+  } else {
+    tmp = umax / d->dx;
+    tmp += vmax / d->dy;
+    tmp += wmax / d->dz;
+  }
   real dts = s->CFL / tmp;
   real mn = FLT_MAX;
   if(dts < mn) mn = dts;
+  if(s->implicit && mn > 1.1*s->dt) mn = 1.1*s->dt;
   return mn;
 }
 
@@ -461,9 +469,11 @@ void orc_store_u(orc_state *s)
   memcpy(s->conv0_u, s->conv_u, fx);
   memcpy(s->conv0_v, s->conv_v, fy);
   memcpy(s->conv0_w, s->conv_w, fz);
-  memcpy(s->diff0_u, s->diff_u, fx);
-  memcpy(s->diff0_v, s->diff_v, fy);
-  memcpy(s->diff0_w, s->diff_w, fz);
+  if(!s->implicit) {
+    memcpy(s->diff0_u, s->diff_u, fx);
+    memcpy(s->diff0_v, s->diff_v, fy);
+    memcpy(s->diff0_w, s->diff_w, fz);
+  }
   memcpy(s->p0, s->p, cc);
   memcpy(s->u0, s->u, fx);
   memcpy(s->v0, s->v, fy);

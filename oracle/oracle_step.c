@@ -53,7 +53,12 @@ int orc_step(orc_state *s)
   int iter = 0;
   real iter_err = FLT_MAX;
   while(iter_err > s->lamb_residual) {
-    orc_U_star_2(s);
+    if(!s->implicit) orc_U_star_2(s);
+    else {
+      orc_star_helmholtz(s, 0);
+      orc_star_helmholtz(s, 1);
+      orc_star_helmholtz(s, 2);
+    }
     if(s->nparts > 0) orc_part_BC_star(s);
     orc_dom_BC_star(s);
     orc_solvability(s);

@@ -50,6 +50,7 @@ class orc_state(C.Structure):
         ("coeff_stride", C.c_int), ("coeff", _PR * 18),
         ("last_iters", C.c_int), ("last_resid", real),
         ("last_converged", C.c_int),
+        ("implicit", C.c_int),
     ]
 
 
@@ -104,6 +105,14 @@ def load():
                                      C.c_double, _PR, _PI]
         lib.orc_pp_matvec.argtypes = [C.POINTER(orc_state), _PR, _PR]
         lib.orc_pp_coeffs.argtypes = [C.POINTER(orc_state), C.c_int, _PR]
+        lib.orc_star_helmholtz.argtypes = [C.POINTER(orc_state), C.c_int]
+        lib.orc_helmholtz_coeffs.argtypes = [C.POINTER(orc_state), C.c_int,
+                                             _PR]
+        lib.orc_helmholtz_matvec.argtypes = [C.POINTER(orc_state), C.c_int,
+                                             _PR, _PR, _PR]
+        lib.orc_pcg.restype = C.c_int
+        lib.orc_pcg.argtypes = [C.POINTER(orc_state), C.c_int, _PR, _PR, _PR,
+                                C.c_int, C.c_double, _PR, _PI]
         lib.orc_lebedev_nodes.argtypes = [_PR, _PR, _PR]
         lib.orc_xyz2rtp.argtypes = [C.c_double] * 3 + [_PR] * 3
         lib.orc_lamb_vel.argtypes = [C.POINTER(orc_state), C.c_int,
@@ -157,6 +166,7 @@ class OracleState:
             setattr(s, k, runtime.get_scalar(k))
         for k in INT_SCALARS:
             setattr(s, k, runtime.get_scalar(k, "i"))
+        s.implicit = 1 if sim.case.implicit else 0
         for name in FIELD_NAMES:
             np.copyto(o.array(name), sim.array(name))
         if n:

@@ -1,0 +1,119 @@
+"""GPU parity of the Crank-Nicolson (IMPLICIT build) path: the three
+cuda_{u,v,w}star_helmholtz solves, find_dt and store_u of that build, and
+whole implicit time steps, against the FP64 oracle.
+
+Each solve starts both sides from the same pulled state.  u* is compared at
+1e-10 relative: both conjugate-gradient runs stop at the same relative
+residual (1e-11 and tighter here) along iterates whose dot products are
+summed in different orders, so the iteration counts may differ by a few
+near the threshold."""
+import dataclasses
+
+import pytest
+
+from bluebottle_amd import configs, sim
+from oracle.pyoracle import OracleState
+
+from conftest import require_particle_kernels
+from helpers import compare_fields, compare_ints, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL_HELM = 1e-10
+
+
+def _implicit(case):
+    return dataclasses.replace(case, implicit=True)
+
+
+CASES = {
+    "tg_free": lambda: _implicit(configs.periodic_case(
+        16, 0.05, CFL=0.3, pp_max_iter=500, pp_residual=1e-12)),
+    "channel": lambda: _implicit(configs.channel_case(n=(16, 12, 16),
+                                                      nparts=2)),
+    "channel_free": lambda: _implicit(configs.channel_case(n=(16, 12, 16),
+                                                           nparts=0)),
+    "settling": lambda: _implicit(configs.small_particle_case(
+        n=24, nparts=3, radius_dx=3.0, taylor_green=False, lamb_max_iter=2)),
+}
+
+
+@pytest.fixture(scope="module", params=sorted(CASES))
+def warmed(request):
+    print("implicit case", request.param, flush=True)
+    case = CASES[request.param]()
+    require_particle_kernels(case)
+    s = sim.Simulation(case)
+    s.to_device()
+    assert s.hip.bb_is_implicit() == 1
+    s.prepare()
+    s.step()
+    s.pull()
+    yield request.param, s
+    s.close()
+
+
+@pytest.mark.parametrize("comp", [0, 1, 2])
+def test_star_helmholtz(warmed, comp):
+    name, s = warmed
+    o = OracleState.from_sim(s)
+    fn = ("cuda_ustar_helmholtz", "cuda_vstar_helmholtz",
+          "cuda_wstar_helmholtz")[comp]
+    getattr(s.hip, fn)(0)
+    o.lib.orc_star_helmholtz(o.ptr, comp)
+    s.pull()
+    st = s.solve_stats()
+    print("%s %s: gpu iters %d resid %.3e | oracle iters %d resid %.3e" % (
+        name, fn, st.iterations, st.residual, o.s.last_iters,
+        o.s.last_resid), flush=True)
+    assert st.converged == 1 and o.s.last_converged == 1
+    star = ("u_star", "v_star", "w_star")[comp]
+    conv = ("conv_u", "conv_v", "conv_w")[comp]
+    compare_fields(s, o, (conv,), 1e-12)
+    e = compare_fields(s, o, (star,), TOL_HELM)
+    print("%s %s: %s rel err %.3e" % (name, fn, star, e[star]))
+    assert abs(st.iterations - o.s.last_iters) <= max(3, 0.05 * o.s.last_iters)
+
+
+def test_find_dt_implicit(warmed):
+    _, s = warmed
+    o = OracleState.from_sim(s)
+    a = s.hip.cuda_find_dt()
+    b = o.lib.orc_find_dt(o.ptr)
+    assert a == b or abs(a - b) <= 1e-15 * abs(b)
+
+
+def test_store_u_implicit(warmed):
+    _, s = warmed
+    o = OracleState.from_sim(s)
+    s.hip.cuda_store_u()
+    o.call("store_u")
+    s.pull()
+    compare_fields(s, o, ("u0", "v0", "w0", "p0", "conv0_u", "conv0_v",
+                          "conv0_w"), 0.0)
+
+
+@pytest.mark.parametrize("case_name", sorted(CASES))
+def test_implicit_full_steps(case_name):
+    print("implicit steps", case_name, flush=True)
+    case = CASES[case_name]()
+    require_particle_kernels(case)
+    s = sim.Simulation(case)
+    s.to_device()
+    s.prepare()
+    s.pull()
+    o = OracleState.from_sim(s)
+    worst = {}
+    for step in range(3):
+        ia = s.step()
+        ib = o.call("step")
+        s.pull()
+        assert ia == ib, "Lamb sub-iterations differ at step %d" % step
+        for f in ("u", "v", "w", "p", "phi"):
+            worst[f] = max(worst.get(f, 0.0), rel_err(s.array(f), o.array(f)))
+        compare_ints(s, o)
+    print("implicit steps %s: %s" % (case_name, {k: "%.2e" % v for k, v in
+                                                  worst.items()}))
+    for f, e in worst.items():
+        assert e <= 1e-8, "%s: %.3e" % (f, e)
+    s.close()

@@ -19,6 +19,7 @@ import pytest
 from bluebottle_amd import configs, sim
 from oracle.pyoracle import OracleState
 
+from conftest import require_particle_kernels
 from helpers import (compare_fields, compare_ints, compare_parts, rel_err,
                      COEFF_SET)
 
@@ -49,6 +50,7 @@ CASES = {
     "tg_particles": lambda: configs.small_particle_case(
         n=24, nparts=2, radius_dx=3.0, taylor_green=True, lamb_max_iter=2),
     "channel": lambda: configs.channel_case(n=(16, 12, 16), nparts=2),
+    "channel_free": lambda: configs.channel_case(n=(16, 12, 16), nparts=0),
     "tg_free": lambda: configs.periodic_case(16, 0.01, CFL=0.3, pp_max_iter=500,
                                              pp_residual=1e-11),
 }
@@ -57,7 +59,9 @@ CASES = {
 @pytest.fixture(scope="module", params=sorted(CASES))
 def warmed(request):
     print("case", request.param, flush=True)
-    s = _warm(CASES[request.param](), 2)
+    case = CASES[request.param]()
+    require_particle_kernels(case)
+    s = _warm(case, 2)
     yield request.param, s
     s.close()
 

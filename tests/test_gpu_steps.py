@@ -7,6 +7,7 @@ import pytest
 from bluebottle_amd import sim
 from oracle.pyoracle import OracleState
 
+from conftest import require_particle_kernels
 from helpers import compare_ints, rel_err
 from test_gpu_parity import CASES, TOL_SOLVE
 
@@ -14,10 +15,12 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("case_name", ["settling", "channel", "tg_free",
-                                       "tg_particles"])
+                                       "tg_particles", "channel_free"])
 def test_full_steps(case_name):
     print("case", case_name, flush=True)
-    s = sim.Simulation(CASES[case_name]())
+    case = CASES[case_name]()
+    require_particle_kernels(case)
+    s = sim.Simulation(case)
     s.to_device()
     s.prepare()
     s.pull()
