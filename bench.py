@@ -249,6 +249,17 @@ def main():
         sys.exit(2)
 
     from bluebottle_amd import configs, runtime, sim
+    if not runtime.gpu_path_enabled():
+        # nothing touches the device: report the state instead of a number
+        print(json.dumps({
+            "metric": METRIC, "value": None, "unit": "timesteps/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": None, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
+            "data": "synthetic", "config": {"workload": "none run"},
+            "roofline": None, "cpu_baseline": None,
+            "error": runtime.GPU_PATH_OFF_REASON}), flush=True)
+        sys.exit(1)
     workload = args.workload
     if workload == "auto":
         workload = ("sediment" if os.environ.get("BB_PARTICLE_KERNELS") == "1"

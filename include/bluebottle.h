@@ -371,6 +371,11 @@ void bb_set_implicit(int mode);
 /* 1 when the allocated domain runs as the IMPLICIT build. */
 int bb_is_implicit(void);
 
+/* Device execution is off unless the environment variable BB_GPU_PATH=1
+ * is set when the first entry point touches the device; otherwise that
+ * entry point prints a message and exits.  Returns 1 when it is on. */
+int bb_gpu_path_enabled(void);
+
 /* The particle boundary-value, internal-velocity, pressure-fill,
  * quadrature and motion kernels (cuda_part_BC, cuda_part_BC_star,
  * cuda_part_p_fill, cuda_parts_internal, cuda_Lamb, cuda_lamb_err,
