@@ -25,6 +25,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 #include "bb_device.h"
 
 #define TX 64
@@ -214,7 +215,7 @@ struct BCBatch {
   int nf;
 };
 
-__device__ inline void bc_apply(double *F, int str, int lo, int type,
+__device__ inline void bc_apply(double *F, long str, int lo, int type,
   int normal, double bcv, int s, int e, int sb, int eb)
 {
   if(type == PERIODIC) {
@@ -239,23 +240,40 @@ __device__ inline void bc_apply(double *F, int str, int lo, int type,
   }
 }
 
-__global__ void __launch_bounds__(256) k_bc_axis(BCBatch B, int axis)
+/* One field's work along one axis, resolved on the host: the kernel reads
+ * every value at a fixed kernel-argument offset, with no per-axis
+ * selection on the device (a three-way selection of argument fields by a
+ * run-time axis index compiled, for gfx950, into loads through scalar
+ * registers that the axis-2 path never set). */
+struct BCAxisField {
+  double *f;
+  long s1, s2;         /* strides of the two transverse axes */
+  long str;            /* stride along the axis */
+  int n1, n2, nax;     /* ghosted extents: transverse, transverse, axis */
+  int normal;          /* 1 when the axis is the field's staggered axis */
+  int type_lo, type_hi;
+  double val_lo, val_hi;
+};
+
+struct BCAxisBatch {
+  BCAxisField fld[4];
+  int nf;
+};
+
+__global__ void __launch_bounds__(256) k_bc_axis(BCAxisBatch B)
 {
   const int t1 = blockIdx.x * blockDim.x + threadIdx.x;
   const int t2 = blockIdx.y * blockDim.y + threadIdx.y;
-  for(int q = 0; q < B.nf; q++) {
-    const BCField &fd = B.fld[q];
-    int n1, n2, str, s1, s2, nax;
-    if(axis == 0) { n1 = fd.n[1]; n2 = fd.n[2]; str = 1; s1 = fd.s1; s2 = fd.s2; nax = fd.n[0]; }
-    else if(axis == 1) { n1 = fd.n[0]; n2 = fd.n[2]; str = fd.s1; s1 = 1; s2 = fd.s2; nax = fd.n[1]; }
-    else { n1 = fd.n[0]; n2 = fd.n[1]; str = fd.s2; s1 = 1; s2 = fd.s1; nax = fd.n[2]; }
-    if(t1 >= n1 || t2 >= n2) continue;
-    double *F = fd.f + (long) t1 * s1 + (long) t2 * s2;
+#pragma unroll
+  for(int q = 0; q < 4; q++) {
+    if(q >= B.nf) break;
+    const BCAxisField &fd = B.fld[q];
+    if(t1 >= fd.n1 || t2 >= fd.n2) continue;
+    double *F = fd.f + (long) t1 * fd.s1 + (long) t2 * fd.s2;
     /* along the axis: sb = 0, s = 1, e = nax-1, eb = nax */
-    int s = 1, e = nax - 1, sb = 0, eb = nax;
-    int normal = (fd.normal == axis);
-    bc_apply(F, str, 1, fd.type[2*axis], normal, fd.val[2*axis], s, e, sb, eb);
-    bc_apply(F, str, 0, fd.type[2*axis+1], normal, fd.val[2*axis+1], s, e, sb, eb);
+    const int s = 1, e = fd.nax - 1, sb = 0, eb = fd.nax;
+    bc_apply(F, fd.str, 1, fd.type_lo, fd.normal, fd.val_lo, s, e, sb, eb);
+    bc_apply(F, fd.str, 0, fd.type_hi, fd.normal, fd.val_hi, s, e, sb, eb);
   }
 }
 
@@ -304,16 +322,39 @@ static BCField bc_field(double *f, char which)
   return fd;
 }
 
+/* The per-axis view of field `fd`: transverse extents and strides in
+ * (x, y, z) order with the axis removed. */
+static BCAxisField bc_axis_field(const BCField &fd, int axis)
+{
+  BCAxisField a;
+  const long st[3] = {1, fd.s1, fd.s2};
+  const int t1 = axis == 0 ? 1 : 0;
+  const int t2 = axis == 2 ? 1 : 2;
+  a.f = fd.f;
+  a.n1 = fd.n[t1]; a.n2 = fd.n[t2]; a.nax = fd.n[axis];
+  a.s1 = st[t1]; a.s2 = st[t2]; a.str = st[axis];
+  a.normal = fd.normal == axis;
+  a.type_lo = fd.type[2*axis]; a.type_hi = fd.type[2*axis+1];
+  a.val_lo = fd.val[2*axis]; a.val_hi = fd.val[2*axis+1];
+  return a;
+}
+
 static void bc_launch(BCBatch &B)
 {
-  int mx = 0;
-  for(int q = 0; q < B.nf; q++)
-    for(int a = 0; a < 3; a++) if(B.fld[q].n[a] > mx) mx = B.fld[q].n[a];
-  dim3 blk(64, 4);
-  dim3 grd((mx + 63) / 64, (mx + 3) / 4);
   bb_prof_begin(BB_K_BC);
   for(int axis = 0; axis < 3; axis++) {
-    k_bc_axis<<<grd, blk, 0, g_bb.stream>>>(B, axis);
+    BCAxisBatch A;
+    memset(&A, 0, sizeof(A));
+    A.nf = B.nf;
+    int m1 = 0, m2 = 0;
+    for(int q = 0; q < B.nf; q++) {
+      A.fld[q] = bc_axis_field(B.fld[q], axis);
+      if(A.fld[q].n1 > m1) m1 = A.fld[q].n1;
+      if(A.fld[q].n2 > m2) m2 = A.fld[q].n2;
+    }
+    dim3 blk(64, 4);
+    dim3 grd((m1 + 63) / 64, (m2 + 3) / 4);
+    k_bc_axis<<<grd, blk, 0, g_bb.stream>>>(A);
     BB_CHECK_LAUNCH();
   }
   bb_prof_end(BB_K_BC);

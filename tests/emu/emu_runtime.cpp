@@ -14,6 +14,8 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -293,13 +295,100 @@ hipError_t hipFree(void *p)
   if(!guarded_free(p)) free(p);
   return hipSuccess;
 }
+static std::set<uintptr_t> g_host_allocs;
+
 hipError_t hipHostMalloc(void **p, size_t n, unsigned)
 {
   *p = malloc(n ? n : 1);
   reg_add(*p, n);
+  {
+    std::lock_guard<std::mutex> l(g_reg_mu);
+    g_host_allocs.insert((uintptr_t) *p);
+  }
   return *p ? hipSuccess : hipErrorOutOfMemory;
 }
-hipError_t hipHostFree(void *p) { reg_del(p); free(p); return hipSuccess; }
+hipError_t hipHostFree(void *p)
+{
+  reg_del(p);
+  {
+    std::lock_guard<std::mutex> l(g_reg_mu);
+    g_host_allocs.erase((uintptr_t) p);
+  }
+  free(p);
+  return hipSuccess;
+}
+
+/* ---- launch capture for the machine-code replay (tests/isa) ---- */
+namespace {
+std::map<std::string, int> g_cap_count;
+int g_cap_seq = 0;
+std::string g_cap_dir;
+
+const char *cap_dir(void)
+{
+  static int init = 0;
+  if(!init) {
+    init = 1;
+    const char *e = getenv("BB_EMU_ISA_DUMP");
+    if(e) g_cap_dir = e;
+  }
+  return g_cap_dir.empty() ? nullptr : g_cap_dir.c_str();
+}
+
+/* device allocations only: pinned host buffers are never kernel operands */
+std::vector<std::pair<uintptr_t, size_t>> cap_allocs(void)
+{
+  std::vector<std::pair<uintptr_t, size_t>> out;
+  std::lock_guard<std::mutex> l(g_reg_mu);
+  for(auto &kv : g_reg)
+    if(!g_host_allocs.count(kv.first)) out.push_back(kv);
+  return out;
+}
+}
+
+namespace bb_emu {
+bool isa_capture_wanted(const char *name)
+{
+  if(!cap_dir()) return false;
+  const char *mx = getenv("BB_EMU_ISA_MAX");
+  int lim = mx ? atoi(mx) : 2;
+  const char *only = getenv("BB_EMU_ISA_KERNELS");
+  if(only && *only) {
+    std::string list = std::string(",") + only + ",";
+    if(list.find(std::string(",") + name + ",") == std::string::npos) return false;
+  }
+  return g_cap_count[name] < lim;
+}
+
+void isa_capture(const char *name, const char *ftype, dim3 g, dim3 b,
+  const std::vector<unsigned char> &args, int phase)
+{
+  static std::string stem;
+  static std::vector<std::pair<uintptr_t, size_t>> al;
+  if(phase == 0) {
+    char buf[512];
+    std::string nm = name;
+    for(auto &c : nm) if(!isalnum((unsigned char) c) && c != '_') c = '_';
+    snprintf(buf, sizeof(buf), "%s/%05d_%s", cap_dir(), g_cap_seq++, nm.c_str());
+    stem = buf;
+    al = cap_allocs();
+    FILE *f = fopen((stem + ".json").c_str(), "w");
+    fprintf(f, "{\"kernel\": \"%s\", \"ftype\": \"%s\", \"grid\": [%u, %u, %u], "
+      "\"block\": [%u, %u, %u], \"args\": \"", name, ftype, g.x, g.y, g.z, b.x, b.y, b.z);
+    for(unsigned char c : args) fprintf(f, "%02x", c);
+    fprintf(f, "\", \"allocs\": [");
+    for(size_t i = 0; i < al.size(); i++)
+      fprintf(f, "%s[%llu, %zu]", i ? ", " : "", (unsigned long long) al[i].first,
+        al[i].second);
+    fprintf(f, "]}\n");
+    fclose(f);
+  }
+  FILE *f = fopen((stem + (phase == 0 ? ".before" : ".after")).c_str(), "wb");
+  for(auto &a : al) fwrite((const void *) a.first, 1, a.second, f);
+  fclose(f);
+  if(phase == 1) g_cap_count[name]++;
+}
+}
 
 static void check_copy(void *d, const void *s, size_t n, hipMemcpyKind k)
 {

@@ -26,7 +26,9 @@
 #include <algorithm>
 #include <functional>
 #include <type_traits>
+#include <typeinfo>
 #include <utility>
+#include <vector>
 
 #define __global__
 #define __device__
@@ -101,6 +103,13 @@ int shfl_down_i(int v, unsigned delta, int width);
 bool is_device_range(const void *p, size_t bytes);
 void report_bad_pointer(const char *kernel, int arg, const void *p);
 void run_grid(const char *name, dim3 g, dim3 b, const std::function<void()> &body);
+/* Launch capture for the machine-code replay (tests/isa): with
+ * BB_EMU_ISA_DUMP=<dir> set, the first launches of each kernel are written
+ * out as (kernel, launch shape, explicit argument bytes in kernarg layout,
+ * every device allocation before and after the launch). */
+bool isa_capture_wanted(const char *name);
+void isa_capture(const char *name, const char *ftype, dim3 g, dim3 b,
+  const std::vector<unsigned char> &args, int phase);
 }
 
 #define __syncthreads() bb_emu::syncthreads()
@@ -158,6 +167,25 @@ inline void check_arg(const char *name, int idx, const A &a)
   }
 }
 
+/* one explicit argument converted to its parameter type P and laid out
+ * at the next multiple of P's alignment, as in the kernarg segment */
+template<typename P, typename A>
+inline void put_param(std::vector<unsigned char> &kb, const A &a)
+{
+  P v = static_cast<P>(a);
+  size_t off = (kb.size() + alignof(P) - 1) / alignof(P) * alignof(P);
+  kb.resize(off + sizeof(P));
+  memcpy(kb.data() + off, &v, sizeof(P));
+}
+
+template<typename... P, typename... Args>
+inline void pack_params(std::vector<unsigned char> &kb, void (*)(P...),
+  const Args &...args)
+{
+  static_assert(sizeof...(P) == sizeof...(Args), "kernel argument count");
+  (put_param<P>(kb, args), ...);
+}
+
 template<typename F>
 struct Launch {
   const char *name;
@@ -169,7 +197,14 @@ struct Launch {
     int idx = 0;
     (check_arg(name, idx++, args), ...);
     F fn = f;
+    const bool cap = isa_capture_wanted(name);
+    std::vector<unsigned char> kb;
+    if(cap) {
+      pack_params(kb, fn, args...);
+      isa_capture(name, typeid(F).name(), g, b, kb, 0);
+    }
     run_grid(name, g, b, [&]() { fn(args...); });
+    if(cap) isa_capture(name, typeid(F).name(), g, b, kb, 1);
   }
 };
 

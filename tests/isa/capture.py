@@ -1,0 +1,48 @@
+"""Capture kernel launches of one case from the host emulation build
+(test infrastructure): every launch of the first few calls of each kernel
+is written as launch shape, kernarg bytes and the device memory image
+before and after, for replay against the gfx950 machine code (replay.py).
+
+    python tests/isa/capture.py <case> <outdir> [steps] [fp64|mixed] [implicit]
+
+Cases are the small parity cases of tests/test_gpu_parity.py; the optional
+fourth argument selects the Poisson solver form and `implicit` the
+Crank-Nicolson (Helmholtz) predictor."""
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+
+
+def main():
+    case_name, out = sys.argv[1], sys.argv[2]
+    steps = int(sys.argv[3]) if len(sys.argv) > 3 else 1
+    mode = {"fp64": 0, "mixed": 1}[sys.argv[4] if len(sys.argv) > 4 else "fp64"]
+    implicit = len(sys.argv) > 5 and sys.argv[5] == "implicit"
+    os.makedirs(out, exist_ok=True)
+    os.environ["BB_EMU"] = "1"
+    os.environ["BB_EMU_ISA_DUMP"] = out
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import conftest  # noqa: F401  (points the loader at the emulation build)
+    from test_gpu_parity import CASES
+    import dataclasses
+    from bluebottle_amd import sim
+    case = CASES[case_name]()
+    if implicit:
+        case = dataclasses.replace(case, implicit=True)
+    s = sim.Simulation(case)
+    s.to_device()
+    s.set_solver(mode)
+    s.prepare()
+    for _ in range(steps):
+        s.step()
+    s.pull()
+    s.close()
+    print("captured", len([f for f in os.listdir(out) if f.endswith(".json")]),
+          "launches of", case_name, flush=True)
+
+
+if __name__ == "__main__":
+    main()

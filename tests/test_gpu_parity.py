@@ -27,6 +27,7 @@ pytestmark = pytest.mark.gpu
 
 TOL_STENCIL = 1e-12
 TOL_SOLVE = 1e-9
+TOL_MIXED = 1e-8
 
 
 def _warm(case, steps):
@@ -112,6 +113,33 @@ def test_pp_bicgstab(warmed):
     # the dots moves that iterate by a few near the threshold (74 against 70
     # on the wall-bounded box at 1e-11)
     assert abs(st.iterations - o.s.last_iters) <= max(6, 0.1 * o.s.last_iters)
+
+
+def test_pp_bicgstab_mixed(warmed):
+    """The mixed form (FP32 BiCGSTAB inside FP64 iterative refinement) is
+    judged by the same true FP64 relative residual as the plain form, so
+    its phi differs from the FP64 oracle's by at most the spread of
+    solutions within that residual: TOL_MIXED relative."""
+    _, s = warmed
+    s.hip.cuda_U_star_2()
+    s.hip.cuda_dom_BC_star()
+    # a cold start, so that the solve does real work on both sides
+    s.pull()
+    s.array("phi")[:] = 0.0
+    s.push()
+    s.set_solver(1)
+    try:
+        o = _stage(s, lambda: s.hip.cuda_PP_bicgstab(0),
+                   lambda o: o.call("PP_bicgstab"))
+    finally:
+        s.set_solver(0)
+    st = s.solve_stats()
+    print("PP mixed: gpu outer %d inner %d resid %.3e | oracle iters %d "
+          "resid %.3e" % (st.outer_steps, st.iterations, st.residual,
+                          o.s.last_iters, o.s.last_resid))
+    assert st.converged and o.s.last_converged
+    e = compare_fields(s, o, ("phi",), TOL_MIXED)
+    print("PP mixed: phi rel err %.3e" % e["phi"])
 
 
 def test_project(warmed):
