@@ -241,6 +241,10 @@ def main():
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--krylov-sample", type=int, default=10)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run the library's plain HBM copy of BYTES "
+                         "(10 launches) and print its rate: the reference for "
+                         "calibrating FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not "
@@ -249,17 +253,12 @@ def main():
         sys.exit(2)
 
     from bluebottle_amd import configs, runtime, sim
-    if not runtime.gpu_path_enabled():
-        # nothing touches the device: report the state instead of a number
-        print(json.dumps({
-            "metric": METRIC, "value": None, "unit": "timesteps/s",
-            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": None, "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
-            "data": "synthetic", "config": {"workload": "none run"},
-            "roofline": None, "cpu_baseline": None,
-            "error": runtime.GPU_PATH_OFF_REASON}), flush=True)
-        sys.exit(1)
+    if args.calibrate_copy:
+        _, hip = runtime.load()
+        gbs = hip.bb_copy_bandwidth(args.calibrate_copy, 10)
+        print(json.dumps({"copy_bytes": args.calibrate_copy, "launches": 10,
+                          "GB/s": round(gbs, 1)}), flush=True)
+        return
     workload = args.workload
     if workload == "auto":
         workload = ("sediment" if os.environ.get("BB_PARTICLE_KERNELS") == "1"
@@ -282,6 +281,9 @@ def main():
     s.hip.bb_synchronize()
     log("set-up done in %.1f s on %s" % (time.perf_counter() - t0,
                                         s.hip.bb_device_name().decode()))
+    # the achievable streaming rate on this device: a plain 1 GiB copy
+    copy_gbs = s.hip.bb_copy_bandwidth(1 << 30, 10)
+    log("plain HBM copy: %.0f GB/s" % copy_gbs)
     xn, yn, zn = case.n
     ncell = xn * yn * zn
     nbytes = kernel_bytes(ncell, xn, yn, zn)
@@ -322,6 +324,7 @@ def main():
                       for k, v in d.items() if k != "prof"}
                   for m, d in modes.items()},
         "roofline": roof,
+        "hbm_copy_GBs": round(copy_gbs, 1),
         "kernels": table,
         "cpu_baseline": cpu,
     }

@@ -218,6 +218,9 @@ struct BCBatch {
 __device__ inline void bc_apply(double *F, long str, int lo, int type,
   int normal, double bcv, int s, int e, int sb, int eb)
 {
+  /* ghost values are bit-identical to the reference's separately rounded
+   * products and sums */
+#pragma clang fp contract(off)
   if(type == PERIODIC) {
     if(normal) {
       if(lo) { F[sb*str] = F[(e-2)*str]; F[s*str] = F[(e-1)*str]; }

@@ -76,7 +76,7 @@ EXTRA_HIP = {
     "bb_set_mixed_params": (None, [C.c_double, C.c_int, C.c_int]),
     "bb_set_implicit": (None, [C.c_int]),
     "bb_is_implicit": (C.c_int, []),
-    "bb_gpu_path_enabled": (C.c_int, []),
+    "bb_copy_bandwidth": (C.c_double, [C.c_long, C.c_int]),
     "bb_enable_particle_kernels": (None, [C.c_int]),
     "bb_particle_kernels_enabled": (C.c_int, []),
 }
@@ -88,17 +88,6 @@ PROF_KINDS = [
     "update_xr32", "update_p32", "project", "update_pressure", "find_dt",
     "cages", "part_bc", "quadrature", "move_parts", "misc",
 ]
-
-
-GPU_PATH_OFF_REASON = (
-    "device execution of libbluebottle_hip is switched off (BB_GPU_PATH=1 "
-    "enables it): every MI355X run of the library so far ended in a GPU "
-    "memory fault whose cause has not been identified")
-
-
-def gpu_path_enabled():
-    """True when BB_GPU_PATH=1: the library may then touch the device."""
-    return os.environ.get("BB_GPU_PATH") == "1"
 
 
 def libs_present():

@@ -371,10 +371,6 @@ void bb_set_implicit(int mode);
 /* 1 when the allocated domain runs as the IMPLICIT build. */
 int bb_is_implicit(void);
 
-/* Device execution is off unless the environment variable BB_GPU_PATH=1
- * is set when the first entry point touches the device; otherwise that
- * entry point prints a message and exits.  Returns 1 when it is on. */
-int bb_gpu_path_enabled(void);
 
 /* The particle boundary-value, internal-velocity, pressure-fill,
  * quadrature and motion kernels (cuda_part_BC, cuda_part_BC_star,
@@ -417,6 +413,12 @@ enum {
 void bb_prof_enable(int on);
 void bb_prof_read(int kind, real *ms, long *count);
 void bb_prof_reset(void);
+
+/* A plain streaming device-to-device copy of `bytes`, `reps` times on the
+ * library stream; returns GB/s counting every byte read and written.  The
+ * reference point for the kernels' HBM rates and for calibrating hardware
+ * traffic counters (one read and one write of `bytes` per launch). */
+real bb_copy_bandwidth(long bytes, int reps);
 
 #ifdef __cplusplus
 }

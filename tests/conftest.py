@@ -18,7 +18,6 @@ if EMU:
     _rt.HIP_LIB = os.path.join(_emu_lib, "libbluebottle_hip.so")
     _rt.HOST_LIB = os.path.join(_emu_lib, "libbluebottle_host.so")
     os.environ["BB_PARTICLE_KERNELS"] = "1"
-    os.environ["BB_GPU_PATH"] = "1"
 
 # The library's particle boundary, quadrature and motion kernels are off
 # unless BB_PARTICLE_KERNELS=1 (their first MI355X runs ended in a GPU
@@ -35,20 +34,6 @@ def require_particle_kernels(case):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
-
-
-def pytest_collection_modifyitems(config, items):
-    """GPU tests skip while the library's device execution is switched off
-    (BB_GPU_PATH unset): its MI355X runs so far ended in a GPU memory fault
-    whose cause has not been identified.  The emulation runs them."""
-    if EMU or os.environ.get("BB_GPU_PATH") == "1":
-        return
-    skip = pytest.mark.skip(reason="device execution switched off until the "
-                            "cause of the MI355X faults is shown "
-                            "(BB_GPU_PATH=1 enables it)")
-    for item in items:
-        if "gpu" in item.keywords:
-            item.add_marker(skip)
 
 
 @pytest.fixture(scope="session")

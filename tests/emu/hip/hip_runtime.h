@@ -43,6 +43,9 @@ struct dim3 {
     : x(a), y(b), z(c) {}
 };
 
+struct double2 { double x, y; };
+inline double2 make_double2(double x, double y) { return double2{x, y}; }
+
 extern thread_local dim3 threadIdx;
 extern thread_local dim3 blockIdx;
 extern dim3 blockDim;
@@ -89,6 +92,7 @@ hipError_t hipEventCreate(hipEvent_t *e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s = 0);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipEventSynchronize(hipEvent_t e);
+hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipGetLastError(void);
 const char *hipGetErrorString(hipError_t e);
 

@@ -28,8 +28,21 @@ def main():
     import conftest  # noqa: F401  (points the loader at the emulation build)
     from test_gpu_parity import CASES
     import dataclasses
-    from bluebottle_amd import sim
-    case = CASES[case_name]()
+    from bluebottle_amd import runtime, sim
+    if case_name == "copy":
+        # the HBM copy reference kernel: 64 KiB, two launches
+        _, hip = runtime.load()
+        hip.bb_copy_bandwidth(1 << 16, 2)
+        print("captured", len([f for f in os.listdir(out) if f.endswith(".json")]),
+              "launches of", case_name, flush=True)
+        return
+    extra = {
+        # 16 x 32 x 16: eight Poisson tiles, so the XCD block remap of the
+        # Krylov sweeps takes its permuting branch (as at 256^3)
+        "tg_remap": lambda: sim.Case(n=(16, 32, 16), nu=0.01, CFL=0.3,
+                                     pp_max_iter=500, pp_residual=1e-11),
+    }
+    case = (extra.get(case_name) or CASES[case_name])()
     if implicit:
         case = dataclasses.replace(case, implicit=True)
     s = sim.Simulation(case)
