@@ -232,12 +232,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--nparts", type=int, default=1000)
-    ap.add_argument("--workload", choices=("auto", "sediment", "tg"),
-                    default="auto",
-                    help="sediment: 256^3 with --nparts spheres (needs the "
-                         "particle kernels, BB_PARTICLE_KERNELS=1); tg: the "
-                         "same box as a sphere-free Taylor-Green flow; auto: "
-                         "sediment when the particle kernels are enabled")
+    ap.add_argument("--workload", choices=("sediment", "tg"),
+                    default="sediment",
+                    help="sediment: 256^3 with --nparts spheres (the metric's "
+                         "configuration); tg: the same box as a sphere-free "
+                         "Taylor-Green flow")
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--krylov-sample", type=int, default=10)
@@ -260,9 +259,6 @@ def main():
                           "GB/s": round(gbs, 1)}), flush=True)
         return
     workload = args.workload
-    if workload == "auto":
-        workload = ("sediment" if os.environ.get("BB_PARTICLE_KERNELS") == "1"
-                    else "tg")
     if workload == "sediment":
         case = configs.config3(n=args.n, nparts=args.nparts)
         desc = ("sedimentation %dx%dx%d periodic, %d spheres r=6dx, "
@@ -270,8 +266,8 @@ def main():
                 % (args.n, args.n, args.n, args.nparts))
     else:
         case = configs.taylor_green(n=args.n, pp_residual=1e-8)
-        desc = ("Taylor-Green %dx%dx%d periodic unit box, Re=100, no spheres "
-                "(particle kernels off on the GPU), Poisson rel tol 1e-8"
+        desc = ("Taylor-Green %dx%dx%d periodic unit box, Re=100, no spheres, "
+                "Poisson rel tol 1e-8"
                 % (args.n, args.n, args.n))
     log("setting up %s" % desc)
     t0 = time.perf_counter()

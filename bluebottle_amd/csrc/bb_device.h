@@ -109,10 +109,6 @@ void bb_ensure_workspace(void);
 void bb_free_workspace(void);
 /* Crank-Nicolson solver workspace (bb_helmholtz.hip) */
 void bb_free_helmholtz(void);
-/* Exits with a message when `entry` would launch the particle boundary,
- * quadrature or motion kernels (nparts > 0) while they are switched off
- * (bb_enable_particle_kernels, BB_PARTICLE_KERNELS). */
-void bb_require_particle_kernels(const char *entry);
 /* Particle workspace sized for nparts and the current grid. */
 void bb_ensure_particle_workspace(void);
 void bb_invalidate_face_lists(void);

@@ -44,34 +44,6 @@ LambNorm bb_lamb_norm(void)
   return L;
 }
 
-/* The Lamb boundary-value, internal-velocity, pressure-fill, quadrature
- * and motion kernels are off unless enabled: their first executions on an
- * MI355X ended in a GPU memory fault whose cause has not been identified
- * (the same kernels run clean under the host emulation with address and
- * undefined-behaviour sanitizers).  They are also the only kernels of the
- * library with register spills and private (scratch) memory. */
-static int g_particle_kernels = -1;
-
-extern "C" void bb_enable_particle_kernels(int on) { g_particle_kernels = on != 0; }
-
-extern "C" int bb_particle_kernels_enabled(void)
-{
-  if(g_particle_kernels < 0) {
-    const char *e = getenv("BB_PARTICLE_KERNELS");
-    g_particle_kernels = e != NULL && e[0] == '1';
-  }
-  return g_particle_kernels;
-}
-
-void bb_require_particle_kernels(const char *entry)
-{
-  if(nparts <= 0 || bb_particle_kernels_enabled()) return;
-  fprintf(stderr, "bluebottle_hip: %s: the particle kernels are switched off "
-    "(set BB_PARTICLE_KERNELS=1 or call bb_enable_particle_kernels(1))\n", entry);
-  fflush(stderr);
-  exit(EXIT_FAILURE);
-}
-
 void bb_ensure_particle_workspace(void)
 {
   bb_ensure_context();
@@ -519,14 +491,12 @@ static void part_bc_launch(double *fu, double *fv, double *fw)
 extern "C" void cuda_part_BC(void)
 {
   bb_ensure_context();
-  bb_require_particle_kernels("cuda_part_BC");
   part_bc_launch(_u[0], _v[0], _w[0]);
 }
 
 extern "C" void cuda_part_BC_star(void)
 {
   bb_ensure_context();
-  bb_require_particle_kernels("cuda_part_BC_star");
   part_bc_launch(_u_star[0], _v_star[0], _w_star[0]);
 }
 
@@ -604,7 +574,6 @@ k_p_fill(BBGeom G, double mu, double nu, double rho_f, gradP_struct gP,
 extern "C" void cuda_part_p_fill(void)
 {
   bb_ensure_context();
-  bb_require_particle_kernels("cuda_part_p_fill");
   if(nparts <= 0) return;
   BBGeom G = bb_geom();
   dim3 blk(64, 4);
@@ -663,7 +632,6 @@ k_internal(BBGeom G, const part_struct *__restrict__ parts,
 extern "C" void cuda_parts_internal(void)
 {
   bb_ensure_context();
-  bb_require_particle_kernels("cuda_parts_internal");
   if(nparts <= 0) return;
   BBGeom G = bb_geom();
   dim3 blk(64, 4);
@@ -1217,13 +1185,11 @@ static void move_common(int final_)
 extern "C" void cuda_move_parts_sub(void)
 {
   bb_ensure_context();
-  bb_require_particle_kernels("cuda_move_parts_sub");
   move_common(0);
 }
 
 extern "C" void cuda_move_parts(void)
 {
   bb_ensure_context();
-  bb_require_particle_kernels("cuda_move_parts");
   move_common(1);
 }

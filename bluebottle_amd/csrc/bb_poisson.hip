@@ -40,12 +40,20 @@
 #define PY 4
 #define PZ 16
 
+/* The two SpMV sweeps march in z over SX x SY columns (one 512-thread
+ * block per column tile and z chunk of SZ planes). */
+#define SX 64
+#define SY 8
+#define SZ 32
+
 struct PPGeom {
   int nx, ny, nz;
   int tx, ty, tz;          /* tiles of PX x PY x PZ */
   int perx, pery, perz;
   double ddx, ddy, ddz;    /* 1/h^2 */
   int nblk;
+  int sx, sy, sz;          /* SpMV tiles of SX x SY x SZ */
+  int snblk;
 };
 
 struct DinvTable { double d[128]; double g[128]; };
@@ -84,6 +92,10 @@ static PPGeom pp_geom(void)
   g.ddy = 1. / (d->dy * d->dy);
   g.ddz = 1. / (d->dz * d->dz);
   g.nblk = g.tx * g.ty * g.tz;
+  g.sx = (g.nx + SX - 1) / SX;
+  g.sy = (g.ny + SY - 1) / SY;
+  g.sz = (g.nz + SZ - 1) / SZ;
+  g.snblk = g.sx * g.sy * g.sz;
   return g;
 }
 
@@ -362,85 +374,297 @@ k_scale2(size_t n, const KrylovScalars *sc, T *__restrict__ x,
   }
 }
 
+/* ---------------------------------------------------------------------
+ * The SpMV sweeps K1 and K2 march in z.  A 512-thread block owns an
+ * SX x SY column tile and SZ consecutive planes.  The operand of A (M p,
+ * or M s with s = r - alpha v) is formed once per cell: the block stages
+ * plane k with its one-cell x/y halo in LDS, while each thread keeps its
+ * column's planes k-1, k, k+1 in registers.  All global loads of an
+ * iteration are for planes k+1 (halo) and k+2 (column), so their latency
+ * overlaps a whole iteration.  A row is summed in the DIA order of
+ * apply_row, from the same values the reference's matrix product sees.
+ * ------------------------------------------------------------------- */
+
+/* The operand of A at a cell: its value m (M p for K1, M s for K2), the
+ * unscaled s (K2's dots) and the cell's operator flags f.  Sweeps load the
+ * raw inputs of a cell one iteration before they form its value, so that
+ * no load is waited on in the iteration that issues it. */
+struct CellVal { double m, s; uint8_t f; };
+
+template<typename T>
+struct OperandP {
+  struct Raw { T p; uint8_t f; };
+  const T *__restrict__ p;
+  __device__ inline Raw load(const uint8_t *__restrict__ fl, long C) const
+  {
+    Raw w;
+    w.p = p[C];
+    w.f = fl[C];
+    return w;
+  }
+  __device__ inline CellVal val(const Raw &w, const double *dinv) const
+  {
+    CellVal c;
+    c.f = w.f;
+    c.m = dinv[w.f & 127] * (double) w.p;
+    c.s = 0.;
+    return c;
+  }
+};
+
+/* s = r - alpha v rounded to T, as the reference's vector operation */
+template<typename T>
+struct OperandS {
+  struct Raw { T r, v; uint8_t f; };
+  const T *__restrict__ r;
+  const T *__restrict__ v;
+  double alpha;
+  __device__ inline Raw load(const uint8_t *__restrict__ fl, long C) const
+  {
+    Raw w;
+    w.r = r[C];
+    w.v = v[C];
+    w.f = fl[C];
+    return w;
+  }
+  __device__ inline CellVal val(const Raw &w, const double *dinv) const
+  {
+    CellVal c;
+    c.f = w.f;
+    c.s = (double) (T) (1. * (double) w.r + -alpha * (double) w.v);
+    c.m = dinv[w.f & 127] * c.s;
+    return c;
+  }
+};
+
+/* Tile of a sweep block: interior cell (i, j) of this thread, planes
+ * [k0, k1) of the block. */
+struct SweepTile { int i, j, k0, k1, tx, ty; bool act; };
+
+__device__ inline SweepTile sweep_tile(const PPGeom &G, int lb)
+{
+  SweepTile t;
+  const int bx = lb % G.sx;
+  const int rest = lb / G.sx;
+  const int by = rest % G.sy;
+  const int bz = rest / G.sy;
+  t.tx = threadIdx.x;
+  t.ty = threadIdx.y;
+  t.i = bx * SX + t.tx;
+  t.j = by * SY + t.ty;
+  t.k0 = bz * SZ;
+  t.k1 = min(t.k0 + SZ, G.nz);
+  t.act = t.i < G.nx && t.j < G.ny;
+  return t;
+}
+
+/* Plane k of the column of cell (i, j): index of its neighbour above
+ * (honours the periodic wrap; any in-range cell where the coefficient is
+ * zero, as in neighbours()). */
+__device__ inline int plane_above(const PPGeom &G, int k)
+{
+  return (k < G.nz - 1) ? k + 1 : (G.perz ? 0 : k);
+}
+
+__device__ inline int plane_below(const PPGeom &G, int k)
+{
+  return (k > 0) ? k - 1 : (G.perz ? G.nz - 1 : k);
+}
+
+/* In-plane neighbour offsets of (i, j) (periodic wraps; 0 where the
+ * coefficient is zero). */
+struct PlaneNbr { int w, e, s, n; };
+
+__device__ inline PlaneNbr plane_nbr(const PPGeom &G, int i, int j)
+{
+  const int s1 = G.nx;
+  PlaneNbr o;
+  o.w = (i > 0) ? -1 : (G.perx ? G.nx - 1 : 0);
+  o.e = (i < G.nx - 1) ? 1 : (G.perx ? -(G.nx - 1) : 0);
+  o.s = (j > 0) ? -s1 : (G.pery ? (G.ny - 1) * s1 : 0);
+  o.n = (j < G.ny - 1) ? s1 : (G.pery ? -(G.ny - 1) * s1 : 0);
+  return o;
+}
+
+/* The one halo cell of the staged plane this thread stages, if any: the
+ * first SX threads take the row south of the tile, the next SX the row
+ * north of its last row, then SY threads the column west of it and SY the
+ * column east of its last column (144 of the 512 threads).  (row, col) is
+ * its place in the staged plane, off its in-plane cell index; threads
+ * without a halo cell get their own column's in-plane index, so that every
+ * thread issues the same loads. */
+struct HaloCell { bool v; int row, col; long off; };
+
+__device__ inline HaloCell halo_cell(const PPGeom &G, const SweepTile &t,
+  long own)
+{
+  HaloCell h = {false, 0, 0, own};
+  const int tid = t.tx + t.ty * SX;
+  const int bx0 = t.i - t.tx, by0 = t.j - t.ty;
+  const int nc = min(SX, G.nx - bx0), nr = min(SY, G.ny - by0);
+  int i = 0, j = 0;
+  if(tid < SX) {
+    if(tid >= nc) return h;
+    i = bx0 + tid; j = by0;
+    h.off = i + (long) j * G.nx + plane_nbr(G, i, j).s;
+    h.row = 0; h.col = tid + 1;
+  } else if(tid < 2 * SX) {
+    const int x = tid - SX;
+    if(x >= nc) return h;
+    i = bx0 + x; j = by0 + nr - 1;
+    h.off = i + (long) j * G.nx + plane_nbr(G, i, j).n;
+    h.row = nr + 1; h.col = x + 1;
+  } else if(tid < 2 * SX + SY) {
+    const int y = tid - 2 * SX;
+    if(y >= nr) return h;
+    i = bx0; j = by0 + y;
+    h.off = i + (long) j * G.nx + plane_nbr(G, i, j).w;
+    h.row = y + 1; h.col = 0;
+  } else if(tid < 2 * SX + 2 * SY) {
+    const int y = tid - 2 * SX - SY;
+    if(y >= nr) return h;
+    i = bx0 + nc - 1; j = by0 + y;
+    h.off = i + (long) j * G.nx + plane_nbr(G, i, j).e;
+    h.row = y + 1; h.col = nc + 1;
+  } else {
+    return h;
+  }
+  h.v = true;
+  return h;
+}
+
+/* One z-marching sweep: for every interior cell of the tile, y = A (op)
+ * at that cell is handed to `out(C, y, s)`.  Iteration k issues the loads
+ * of the column plane above k+1 and of plane k+1's halo cell, forms the
+ * values loaded by iteration k-1, stages plane k in one of two LDS buffers
+ * and applies the row.  Every thread issues the same loads in the same
+ * order (threads outside the grid or without a halo cell load in-range
+ * cells they do not use), so that the compiler waits only for the loads of
+ * the previous iteration. */
+template<typename Op, typename Out>
+__device__ inline void sweep_z(const PPGeom &G, const SweepTile &t,
+  const uint8_t *__restrict__ fl, const double *dinv,
+  double (*pl)[SY + 2][SX + 2], const Op &op, Out &out)
+{
+  typedef typename Op::Raw Raw;
+  if(t.k0 >= t.k1) return;   /* uniform over the block */
+  const long s2 = (long) G.nx * G.ny;
+  const long col = min(t.i, G.nx - 1) + (long) min(t.j, G.ny - 1) * G.nx;
+  const HaloCell h = halo_cell(G, t, col);
+  Raw raw_c = op.load(fl, col + t.k0 * s2);
+  Raw raw_b = op.load(fl, col + (long) plane_below(G, t.k0) * s2);
+  Raw raw_a = op.load(fl, col + (long) plane_above(G, t.k0) * s2);
+  Raw hraw = op.load(fl, h.off + t.k0 * s2);
+  CellVal cur = op.val(raw_c, dinv);
+  double m_prev = op.val(raw_b, dinv).m;
+  for(int k = t.k0; k < t.k1; k++) {
+    /* on the last plane these reload plane k: in range, unused */
+    const int kn = min(k + 1, t.k1 - 1);
+    const Raw raw_nn = op.load(fl, col + (long) plane_above(G, kn) * s2);
+    const Raw hraw_n = op.load(fl, h.off + (long) kn * s2);
+    const CellVal above = op.val(raw_a, dinv);
+    const double hm = op.val(hraw, dinv).m;
+    double (*P)[SX + 2] = pl[k & 1];
+    if(t.act) P[t.ty + 1][t.tx + 1] = cur.m;
+    if(h.v) P[h.row][h.col] = hm;
+    __syncthreads();
+    if(t.act) {
+      const double y = apply_row(cur.f, G, cur.m,
+        P[t.ty + 1][t.tx], P[t.ty + 1][t.tx + 2],
+        P[t.ty][t.tx + 1], P[t.ty + 2][t.tx + 1],
+        m_prev, above.m, diag_of(dinv, cur.f));
+      out(col + (long) k * s2, y, cur.s);
+    }
+    m_prev = cur.m; cur = above;
+    raw_a = raw_nn; hraw = hraw_n;
+  }
+}
+
+__device__ inline void load_dinv(const DinvTable &tab, double *dinv)
+{
+  const int tid = threadIdx.x + threadIdx.y * SX;
+  if(tid < 256) dinv[tid] = tid < 128 ? tab.d[tid] : tab.g[tid - 128];
+  __syncthreads();
+}
+
+/* Block sum over the 512 threads of a sweep block; valid in thread 0. */
+__device__ inline double sweep_block_sum(double v, double *sh)
+{
+  return bb_block_sum(v, sh);
+}
+
+template<typename T>
+struct OutP {
+  const T *__restrict__ rs;
+  T *__restrict__ v;
+  double acc;
+  __device__ inline void operator()(long C, double y, double) {
+    const T vt = (T) y;
+    v[C] = vt;
+    acc += (double) rs[C] * (double) vt;
+  }
+};
+
+template<typename T>
+struct OutS {
+  T *__restrict__ t;
+  double ts, tt, ss;
+  __device__ inline void operator()(long C, double y, double s) {
+    const T tv = (T) y;
+    t[C] = tv;
+    ts += (double) tv * s;
+    tt += (double) tv * (double) tv;
+    ss += s * s;
+  }
+};
+
 /* K1: v = A M p, partial <r*, v> */
 template<typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(SX * SY)
 k_spmv_p(PPGeom G, DinvTable tab, const KrylovScalars *sc,
   const T *__restrict__ p, const T *__restrict__ rs,
   const uint8_t *__restrict__ fl, T *__restrict__ v,
   double *__restrict__ part)
 {
   if(sc->done) return;
-  __shared__ double sh[4];
+  __shared__ double sh[SX * SY / 64];
   __shared__ double dinv[256];
-  const int tid = threadIdx.x + threadIdx.y * PX;
-  dinv[tid] = tid < 128 ? tab.d[tid] : tab.g[tid - 128];
-  __syncthreads();
-  const int lb = xcd_block(G.nblk);
-  TileIdx Ti = tile_of(G, lb);
-  double acc = 0.;
-  if(Ti.i < G.nx && Ti.j < G.ny) {
-    for(int k = Ti.k0; k < Ti.k1; k++) {
-      const int C = Ti.i + Ti.j*G.nx + k*G.nx*G.ny;
-      Nbr o = neighbours(G, Ti.i, Ti.j, k);
-      uint8_t f = fl[C];
-      double vv = apply_row(f, G, mval(p, fl, dinv, C),
-        mval(p, fl, dinv, C+o.w), mval(p, fl, dinv, C+o.e),
-        mval(p, fl, dinv, C+o.s), mval(p, fl, dinv, C+o.n),
-        mval(p, fl, dinv, C+o.b), mval(p, fl, dinv, C+o.t),
-        diag_of(dinv, f));
-      T vt = (T) vv;
-      v[C] = vt;
-      acc += (double) rs[C] * (double) vt;
-    }
-  }
-  double t = bb_block_sum(acc, sh);
-  if(tid == 0) part[lb] = t;
+  __shared__ double pl[2][SY + 2][SX + 2];
+  load_dinv(tab, dinv);
+  const int lb = xcd_block(G.snblk);
+  const SweepTile t = sweep_tile(G, lb);
+  OperandP<T> op{p};
+  OutP<T> out{rs, v, 0.};
+  sweep_z(G, t, fl, dinv, pl, op, out);
+  const double a = sweep_block_sum(out.acc, sh);
+  if(threadIdx.x == 0 && threadIdx.y == 0) part[lb] = a;
 }
 
-/* K2: s = r - alpha v (recomputed at the neighbours), t = A M s,
+/* K2: s = r - alpha v (formed once per cell), t = A M s,
  * partials <t,s>, <t,t>, <s,s> */
 template<typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(SX * SY)
 k_spmv_s(PPGeom G, DinvTable tab, const KrylovScalars *sc,
   const T *__restrict__ r, const T *__restrict__ v,
-  const uint8_t *__restrict__ fl, T *__restrict__ t,
+  const uint8_t *__restrict__ fl, T *__restrict__ t_out,
   double *__restrict__ part)
 {
   if(sc->done) return;
-  __shared__ double sh[4];
+  __shared__ double sh[SX * SY / 64];
   __shared__ double dinv[256];
-  const int tid = threadIdx.x + threadIdx.y * PX;
-  dinv[tid] = tid < 128 ? tab.d[tid] : tab.g[tid - 128];
-  __syncthreads();
-  const int lb = xcd_block(G.nblk);
-  TileIdx Ti = tile_of(G, lb);
-  const double alpha = sc->alpha;
-  double ts = 0., tt = 0., ss = 0.;
-  if(Ti.i < G.nx && Ti.j < G.ny) {
-    for(int k = Ti.k0; k < Ti.k1; k++) {
-      const int C = Ti.i + Ti.j*G.nx + k*G.nx*G.ny;
-      Nbr o = neighbours(G, Ti.i, Ti.j, k);
-      uint8_t f = fl[C];
-#define SVAL(I) ((T) (1. * (double) r[I] + -alpha * (double) v[I]))
-#define MS(I) (dinv[fl[I] & 127] * (double) SVAL(I))
-      double sc_ = (double) SVAL(C);
-      double tv = apply_row(f, G, dinv[f & 127] * sc_,
-        MS(C+o.w), MS(C+o.e), MS(C+o.s), MS(C+o.n), MS(C+o.b), MS(C+o.t),
-        diag_of(dinv, f));
-#undef SVAL
-#undef MS
-      T tt_ = (T) tv;
-      t[C] = tt_;
-      ts += (double) tt_ * sc_;
-      tt += (double) tt_ * (double) tt_;
-      ss += sc_ * sc_;
-    }
+  __shared__ double pl[2][SY + 2][SX + 2];
+  load_dinv(tab, dinv);
+  const int lb = xcd_block(G.snblk);
+  const SweepTile t = sweep_tile(G, lb);
+  OperandS<T> op{r, v, sc->alpha};
+  OutS<T> out{t_out, 0., 0., 0.};
+  sweep_z(G, t, fl, dinv, pl, op, out);
+  const double a = sweep_block_sum(out.ts, sh);
+  const double b = sweep_block_sum(out.tt, sh);
+  const double c = sweep_block_sum(out.ss, sh);
+  if(threadIdx.x == 0 && threadIdx.y == 0) {
+    part[3*lb] = a; part[3*lb+1] = b; part[3*lb+2] = c;
   }
-  double a = bb_block_sum(ts, sh);
-  double b = bb_block_sum(tt, sh);
-  double c = bb_block_sum(ss, sh);
-  if(tid == 0) { part[3*lb] = a; part[3*lb+1] = b; part[3*lb+2] = c; }
 }
 
 /* K3: x += alpha M p + omega M s; r = s - omega t; partials <r*,r>, <r,r>.
@@ -625,8 +849,14 @@ static void bicgstab_run(const PPGeom &G, const DinvTable &tab, T *b, T *x,
   BBWorkspace *ws = &g_bb.ws;
   hipStream_t st = g_bb.stream;
   dim3 blk(PX, PY);
+  dim3 sblk(SX, SY);
   size_t n = ws->n;
   int eb = elementwise_blocks(n);
+  if(G.snblk > ws->nblk) {
+    fprintf(stderr, "bluebottle_hip: %d sweep tiles exceed the %d reduction "
+      "slots\n", G.snblk, ws->nblk);
+    exit(EXIT_FAILURE);
+  }
   k_init<T><<<G.nblk, blk, 0, st>>>(G, tab, ws->sc, b, x, ws->flags, r, rs,
     p, v, ws->partials, scale_inputs);
   BB_CHECK_LAUNCH();
@@ -638,18 +868,18 @@ static void bicgstab_run(const PPGeom &G, const DinvTable &tab, T *b, T *x,
   BB_CHECK_LAUNCH();
   for(int it = 0; it < max_iter + 1; it++) {
     bb_prof_begin(kinds[0]);
-    k_spmv_p<T><<<G.nblk, blk, 0, st>>>(G, tab, ws->sc, p, rs, ws->flags, v,
-      ws->partials);
+    k_spmv_p<T><<<G.snblk, sblk, 0, st>>>(G, tab, ws->sc, p, rs, ws->flags,
+      v, ws->partials);
     BB_CHECK_LAUNCH();
     bb_prof_end(kinds[0]);
-    k_f1<<<1, 256, 0, st>>>(G.nblk, ws->partials, ws->sc);
+    k_f1<<<1, 256, 0, st>>>(G.snblk, ws->partials, ws->sc);
     BB_CHECK_LAUNCH();
     bb_prof_begin(kinds[1]);
-    k_spmv_s<T><<<G.nblk, blk, 0, st>>>(G, tab, ws->sc, r, v, ws->flags, t,
-      ws->partials);
+    k_spmv_s<T><<<G.snblk, sblk, 0, st>>>(G, tab, ws->sc, r, v, ws->flags,
+      t, ws->partials);
     BB_CHECK_LAUNCH();
     bb_prof_end(kinds[1]);
-    k_f2<<<1, 256, 0, st>>>(G.nblk, ws->partials, ws->sc);
+    k_f2<<<1, 256, 0, st>>>(G.snblk, ws->partials, ws->sc);
     BB_CHECK_LAUNCH();
     bb_prof_begin(kinds[2]);
     k_update_xr<T><<<G.nblk, blk, 0, st>>>(G, tab, ws->sc, x, p, r, v, t, rs,

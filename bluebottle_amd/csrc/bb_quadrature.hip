@@ -237,7 +237,6 @@ extern "C" void cuda_quad_check_nodes(int dev, real *node_t, real *node_p,
   (void) dev;
   bb_ensure_context();
   if(nparts <= 0) return;
-  bb_require_particle_kernels("cuda_quad_check_nodes");
   k_check_nodes<<<nparts, 64, 0, g_bb.stream>>>(bb_geom(), bc, nnodes,
     _parts[0], node_t, node_p);
   BB_CHECK_LAUNCH();
@@ -249,7 +248,6 @@ extern "C" void cuda_quad_interp(int dev, real *node_t, real *node_p,
   (void) dev;
   bb_ensure_context();
   if(nparts <= 0) return;
-  bb_require_particle_kernels("cuda_quad_interp");
   k_interp_nodes<<<nparts, 64, 0, g_bb.stream>>>(interp_args(), nnodes,
     _parts[0], node_t, node_p, _p[0], _u[0], _v[0], _w[0], pp, ur, ut, up);
   BB_CHECK_LAUNCH();
@@ -373,7 +371,6 @@ extern "C" void cuda_Lamb(void)
 {
   bb_ensure_particle_workspace();
   if(nparts <= 0) return;
-  bb_require_particle_kernels("cuda_Lamb");
   CoeffOut K = {_pnm_re[0], _pnm_im[0], _phinm_re[0], _phinm_im[0],
     _chinm_re[0], _chinm_im[0], _pnm_re0[0], _pnm_im0[0], _phinm_re0[0],
     _phinm_im0[0], _chinm_re0[0], _chinm_im0[0]};
@@ -432,7 +429,6 @@ extern "C" real cuda_lamb_err(void)
 {
   bb_ensure_particle_workspace();
   if(nparts <= 0) return 0.;
-  bb_require_particle_kernels("cuda_lamb_err");
   hipStream_t st = g_bb.stream;
   bb_prof_begin(BB_K_QUAD);
   k_lamb_err<<<nparts, 64, 0, st>>>(coeff_stride, lamb_cut, _pnm_re[0],

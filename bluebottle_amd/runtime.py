@@ -77,8 +77,6 @@ EXTRA_HIP = {
     "bb_set_implicit": (None, [C.c_int]),
     "bb_is_implicit": (C.c_int, []),
     "bb_copy_bandwidth": (C.c_double, [C.c_long, C.c_int]),
-    "bb_enable_particle_kernels": (None, [C.c_int]),
-    "bb_particle_kernels_enabled": (C.c_int, []),
 }
 
 # kernel kinds of the event timer (bb_device.h)

@@ -372,15 +372,6 @@ void bb_set_implicit(int mode);
 int bb_is_implicit(void);
 
 
-/* The particle boundary-value, internal-velocity, pressure-fill,
- * quadrature and motion kernels (cuda_part_BC, cuda_part_BC_star,
- * cuda_part_p_fill, cuda_parts_internal, cuda_Lamb, cuda_lamb_err,
- * cuda_quad_*, cuda_move_parts*) are off by default: called with
- * nparts > 0 they print a message and exit.  on = 1, or the environment
- * variable BB_PARTICLE_KERNELS=1 at the first call, enables them.
- * Cages, phases and flags (cuda_build_cages) are not affected. */
-void bb_enable_particle_kernels(int on);
-int bb_particle_kernels_enabled(void);
 
 /* Blocks until all work the library queued has finished. */
 void bb_synchronize(void);

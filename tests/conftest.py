@@ -17,20 +17,6 @@ if EMU:
     _emu_lib = os.path.join(ROOT, "tests", "emu", "_build", "lib")
     _rt.HIP_LIB = os.path.join(_emu_lib, "libbluebottle_hip.so")
     _rt.HOST_LIB = os.path.join(_emu_lib, "libbluebottle_host.so")
-    os.environ["BB_PARTICLE_KERNELS"] = "1"
-
-# The library's particle boundary, quadrature and motion kernels are off
-# unless BB_PARTICLE_KERNELS=1 (their first MI355X runs ended in a GPU
-# memory fault whose cause is not identified); cases with particles are
-# skipped on the GPU until then.  The emulation runs them.
-PARTICLE_KERNELS = os.environ.get("BB_PARTICLE_KERNELS") == "1"
-
-
-def require_particle_kernels(case):
-    if case.particles and not PARTICLE_KERNELS:
-        pytest.skip("particle kernels are switched off on the GPU "
-                    "(BB_PARTICLE_KERNELS=1 enables them)")
-
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")

@@ -37,9 +37,10 @@ def main():
               "launches of", case_name, flush=True)
         return
     extra = {
-        # 16 x 32 x 16: eight Poisson tiles, so the XCD block remap of the
-        # Krylov sweeps takes its permuting branch (as at 256^3)
-        "tg_remap": lambda: sim.Case(n=(16, 32, 16), nu=0.01, CFL=0.3,
+        # 16 x 64 x 16: eight SpMV tiles and 16 element-wise tiles, so
+        # the XCD block remap of the Krylov sweeps takes its permuting
+        # branch (as at 256^3)
+        "tg_remap": lambda: sim.Case(n=(16, 64, 16), nu=0.01, CFL=0.3,
                                      pp_max_iter=500, pp_residual=1e-11),
     }
     case = (extra.get(case_name) or CASES[case_name])()

@@ -223,6 +223,30 @@ _REG = re.compile(r"^([vsa])(?:\[(\d+):(\d+)\]|(\d+))$")
 POISON = 0xBAD0BAD1
 
 
+def _two_over_pi_bits(nbits):
+    """floor(2/pi * 2**nbits) as an integer (pi by Machin's formula)."""
+    guard = 64
+    one = 1 << (nbits + guard)
+
+    def arctan_inv(x):
+        t = one // x
+        s = t
+        x2 = x * x
+        n, sign = 3, -1
+        while t:
+            t //= x2
+            s += sign * (t // n)
+            sign, n = -sign, n + 2
+        return s
+    pi = 4 * (4 * arctan_inv(5) - arctan_inv(239))   # pi * one
+    return (2 * one * (1 << nbits)) // pi
+
+
+# v_trig_preop_f64 reads 53-bit segments of the first 1201 bits of 2/pi
+TWO_OVER_PI_NBITS = 1201 + 53 + 128
+TWO_OVER_PI = _two_over_pi_bits(TWO_OVER_PI_NBITS)
+
+
 class Wave:
     def __init__(self, nsgpr=112, nvgpr=512):
         # registers the dispatch does not initialise hold a poison value, so
@@ -238,6 +262,7 @@ class Wave:
         self.pc = 0
         self.done = False
         self.at_barrier = False
+        self.priv = None
 
 
 def split_ops(rest):
@@ -542,6 +567,8 @@ class Exec:
             return self.gmem(w, op, ops, mods, where)
         if op.startswith("ds_"):
             return self.dsmem(w, op, ops, mods, where)
+        if op.startswith("scratch_"):
+            return self.pmem(w, op, ops, mods, where)
         raise NotImplementedError(op)
 
     def salu(self, w, op, ops, mods, tgt, where):
@@ -599,6 +626,11 @@ class Exec:
             w.exec = new & M64
             w.scc = int(w.exec != 0)
             return
+        if op in ("s_bitcmp0_b32", "s_bitcmp1_b32"):
+            a = self.sv(w, ops[0])
+            bit = (a >> (self.sv(w, ops[1]) & 31)) & 1
+            w.scc = int(bit == (0 if op == "s_bitcmp0_b32" else 1))
+            return
         if op.startswith("s_cmp"):
             return self.scmp(w, op, ops)
         return self.salu_arith(w, op, ops, mods)
@@ -650,6 +682,11 @@ class Exec:
                 b = s32(self.sv(w, ops[2]))
             r = a + b if op != "s_sub_i32" else a - b
             w.scc = int(r > 0x7FFFFFFF or r < -0x80000000)
+            return self.sset(w, ops[0], r & M32)
+        if op == "s_mulk_i32":
+            b = int(ops[1], 0) & 0xFFFF
+            b = b - 0x10000 if b & 0x8000 else b
+            r = s32(self.sv(w, ops[0])) * b
             return self.sset(w, ops[0], r & M32)
         if op == "s_mul_i32":
             r = s32(self.sv(w, ops[1])) * s32(self.sv(w, ops[2]))
@@ -1166,6 +1203,33 @@ class Exec:
         special = ~np.isfinite(q) | ~np.isfinite(d) | ~np.isfinite(n) | (d == 0)
         self.vset(w, o[0], u64_from_f64(np.where(special, exact, q)), 2)
 
+    def op_trig_preop_f64(self, w, o, m, act):
+        """53-bit segment S1[4:0] of 2/pi, skipping exponent(S0) - 1077
+        further bits for large S0, scaled to its place value."""
+        a = self._u(w, o[1], 2)
+        seg = self._u(w, o[2]) & np.uint32(31)
+        out = np.zeros(LANES, dtype=np.uint64)
+        for l in np.nonzero(act)[0]:
+            e = int((int(a[l]) >> 52) & 0x7FF)
+            shift = int(seg[l]) * 53
+            if e > 1077:
+                shift += e - 1077
+            c = (TWO_OVER_PI >> (TWO_OVER_PI_NBITS - shift - 53)) & ((1 << 53) - 1)
+            scale = -53 - shift
+            if e >= 1968:
+                scale += 128
+            out[l] = u64_from_f64(np.ldexp(np.float64(c), scale))
+        self.vset(w, o[0], out, 2)
+
+    def op_mad_i32_i24(self, w, o, m, act):
+        def s24(x):
+            x = x.astype(np.int64) & 0xFFFFFF
+            return np.where(x & 0x800000, x - 0x1000000, x)
+        a = s24(self._u(w, o[1]))
+        b = s24(self._u(w, o[2]))
+        c = self._u(w, o[3]).astype(np.int64)
+        self.vset(w, o[0], ((a * b + c) & M32).astype(np.uint32))
+
     def op_ldexp_f64(self, w, o, m, act):
         a = self.fget(w, o[1])
         e = self._u(w, o[2]).astype(np.int32)
@@ -1346,6 +1410,47 @@ class Exec:
             out[l] = old
         if dst:
             self.vset(w, dst, out)
+
+    def pmem(self, w, op, ops, mods, where):
+        """scratch_{load,store}_*: per-lane private memory of the kernel's
+        private segment size; lane offset = VGPR (or 0) + SGPR (or 0) +
+        instruction offset."""
+        act = mask_to_bool(w.exec)
+        sizes = {"dword": 4, "dwordx2": 8, "dwordx3": 12, "dwordx4": 16}
+        m = re.match(r"scratch_(load|store)_(\w+)$", op)
+        if not m or m.group(2) not in sizes:
+            raise NotImplementedError(op)
+        kind, t = m.groups()
+        n = sizes[t]
+        if w.priv is None:
+            w.priv = [bytearray([0xBA]) * max(self.k.private_size, 0)
+                      for _ in range(LANES)]
+        imm = int(mods.get("offset", "0"), 0)
+        if kind == "load":
+            dst, va, sa = ops[0], ops[1], ops[2]
+        else:
+            va, dst, sa = ops[0], ops[1], ops[2]
+        voff = self._u(w, va).astype(np.int64) if va != "off" \
+            else np.zeros(LANES, dtype=np.int64)
+        soff = 0
+        if sa != "off":
+            self.check_sdef(w, sa, where, "scratch-offset")
+            soff = self.sreg(w, sa, 1)
+            soff = soff - (1 << 32) if soff & (1 << 31) else soff
+        d = _REG.match(dst)
+        lo = int(d.group(2) if d.group(2) else d.group(4))
+        for l in np.nonzero(act)[0]:
+            a = int(voff[l]) + soff + imm
+            mem = w.priv[l]
+            if a < 0 or a + n > len(mem):
+                self.mem.violations.append((where + (int(l),), "private-" + kind, a, n))
+                continue
+            if kind == "load":
+                for i in range(n // 4):
+                    w.v[lo + i][l] = int.from_bytes(mem[a + 4 * i:a + 4 * i + 4], "little")
+            else:
+                for i in range(n // 4):
+                    mem[a + 4 * i:a + 4 * i + 4] = int(w.v[lo + i][l]).to_bytes(4, "little")
 
     def dsmem(self, w, op, ops, mods, where):
         act = mask_to_bool(w.exec)

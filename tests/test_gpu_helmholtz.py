@@ -14,7 +14,6 @@ import pytest
 from bluebottle_amd import configs, sim
 from oracle.pyoracle import OracleState
 
-from conftest import require_particle_kernels
 from helpers import compare_fields, compare_ints, rel_err
 
 pytestmark = pytest.mark.gpu
@@ -42,7 +41,6 @@ CASES = {
 def warmed(request):
     print("implicit case", request.param, flush=True)
     case = CASES[request.param]()
-    require_particle_kernels(case)
     s = sim.Simulation(case)
     s.to_device()
     assert s.hip.bb_is_implicit() == 1
@@ -97,7 +95,6 @@ def test_store_u_implicit(warmed):
 def test_implicit_full_steps(case_name):
     print("implicit steps", case_name, flush=True)
     case = CASES[case_name]()
-    require_particle_kernels(case)
     s = sim.Simulation(case)
     s.to_device()
     s.prepare()

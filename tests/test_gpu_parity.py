@@ -19,7 +19,6 @@ import pytest
 from bluebottle_amd import configs, sim
 from oracle.pyoracle import OracleState
 
-from conftest import require_particle_kernels
 from helpers import (compare_fields, compare_ints, compare_parts, rel_err,
                      COEFF_SET)
 
@@ -61,7 +60,6 @@ CASES = {
 def warmed(request):
     print("case", request.param, flush=True)
     case = CASES[request.param]()
-    require_particle_kernels(case)
     s = _warm(case, 2)
     yield request.param, s
     s.close()

@@ -7,7 +7,6 @@ import pytest
 from bluebottle_amd import sim
 from oracle.pyoracle import OracleState
 
-from conftest import require_particle_kernels
 from helpers import compare_ints, rel_err
 from test_gpu_parity import CASES, TOL_SOLVE
 
@@ -19,7 +18,6 @@ pytestmark = pytest.mark.gpu
 def test_full_steps(case_name):
     print("case", case_name, flush=True)
     case = CASES[case_name]()
-    require_particle_kernels(case)
     s = sim.Simulation(case)
     s.to_device()
     s.prepare()
