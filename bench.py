@@ -49,14 +49,12 @@ def kernel_bytes(n, xn, yn, zn):
         "u_star": 180 * n,
         # u*, v*, w*, phase, phase_shell, flag_u/v/w, phi -> rhs, x, flags
         "pp_setup": (24 + 8 + 12 + 8 + 8 + 8 + 1) * n,
-        # p, r*, flags -> v
-        "spmv_p": (3 * 8 + 1) * n, "spmv_p32": (3 * 4 + 1) * n,
+        # r, p, v, r*, flags -> p, v  (the p update folded into the sweep)
+        "spmv_p": (6 * 8 + 1) * n, "spmv_p32": (6 * 4 + 1) * n,
         # r, v, flags -> t
         "spmv_s": (3 * 8 + 1) * n, "spmv_s32": (3 * 4 + 1) * n,
         # x, p, r, v, t, r*, flags -> x, r
         "update_xr": (8 * 8 + 1) * n, "update_xr32": (8 * 4 + 1) * n,
-        # r, p, v -> p
-        "update_p": 4 * 8 * n, "update_p32": 4 * 4 * n,
         # b, x, flags -> r (FP64)
         "resid64": (3 * 8 + 1) * n,
         # x, e -> x
@@ -139,6 +137,18 @@ def roofline(prof, nbytes):
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "avg_us": round(avg * 1e3, 2),
             "bytes_per_launch": nbytes[name]}, table
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed counter summary of
+    this workload (two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE,
+    calibrated on the library's copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "sediment256_hbm_traffic.json")
+    if workload != "sediment" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f).get("bytes_per_launch", {}).get(kernel)
+    return rec["total"] if rec else None
 
 
 def cpu_baseline(s, fp64, krylov_sample, desc):
@@ -240,6 +250,12 @@ def main():
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--krylov-sample", type=int, default=10)
+    ap.add_argument("--mixed-inner-tol", type=float, default=0.0,
+                    help="relative tolerance of each FP32 inner solve of the "
+                         "mixed form (default: the library's)")
+    ap.add_argument("--poll", type=int, default=0,
+                    help="BiCGSTAB iterations between host convergence checks "
+                         "(default: the library's)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run the library's plain HBM copy of BYTES "
                          "(10 launches) and print its rate: the reference for "
@@ -269,35 +285,46 @@ def main():
         desc = ("Taylor-Green %dx%dx%d periodic unit box, Re=100, no spheres, "
                 "Poisson rel tol 1e-8"
                 % (args.n, args.n, args.n))
-    log("setting up %s" % desc)
-    t0 = time.perf_counter()
-    s = sim.Simulation(case)
-    s.to_device()
-    s.prepare()
-    s.hip.bb_synchronize()
-    log("set-up done in %.1f s on %s" % (time.perf_counter() - t0,
-                                        s.hip.bb_device_name().decode()))
-    # the achievable streaming rate on this device: a plain 1 GiB copy
-    copy_gbs = s.hip.bb_copy_bandwidth(1 << 30, 10)
-    log("plain HBM copy: %.0f GB/s" % copy_gbs)
     xn, yn, zn = case.n
     ncell = xn * yn * zn
     nbytes = kernel_bytes(ncell, xn, yn, zn)
     modes = {}
     names = {"fp64": 0, "mixed": 1}
+    cpu = None
+    copy_gbs = None
     for m in args.modes.split(","):
+        # every solver form starts from the same freshly set-up state
+        log("setting up %s" % desc)
+        t0 = time.perf_counter()
+        s = sim.Simulation(case)
+        if args.mixed_inner_tol > 0:
+            s.hip.bb_set_mixed_params(args.mixed_inner_tol, 0, 0)
+        if args.poll > 0:
+            s.hip.bb_set_poll_interval(args.poll)
+        s.to_device()
+        s.prepare()
+        s.hip.bb_synchronize()
+        log("set-up done in %.1f s on %s" % (time.perf_counter() - t0,
+                                            s.hip.bb_device_name().decode()))
+        if copy_gbs is None:
+            # the achievable streaming rate on this device: a plain 1 GiB copy
+            copy_gbs = s.hip.bb_copy_bandwidth(1 << 30, 10)
+            log("plain HBM copy: %.0f GB/s" % copy_gbs)
         modes[m] = run_mode(s, names[m], args.steps, args.warmup, runtime)
         log("%s: %.4f timesteps/s, %.2f Lamb it/step, %.1f Krylov it/solve" % (
             m, modes[m]["timesteps_per_s"], modes[m]["lamb_iters_per_step"],
             modes[m]["krylov_iters_per_solve"]))
+        if m == "fp64" and not args.no_cpu:
+            log("timing the CPU oracle sample")
+            cpu = cpu_baseline(s, modes["fp64"], args.krylov_sample, desc)
+            log("cpu sample done")
     head = "mixed" if "mixed" in modes else list(modes)[0]
     H = modes[head]
     roof, table = roofline(H["prof"], nbytes)
-    cpu = None
-    if not args.no_cpu and "fp64" in modes:
-        log("timing the CPU oracle sample")
-        cpu = cpu_baseline(s, modes["fp64"], args.krylov_sample, desc)
-        log("cpu sample done")
+    if roof is not None and args.n == 256 and args.nparts == 1000:
+        roof["traffic"] = measured_traffic(roof["kernel"], workload)
+        if roof["traffic"] is not None:
+            roof["traffic_source"] = "profiles/sediment256_hbm_traffic.md"
     out = {
         "metric": METRIC,
         "value": round(H["timesteps_per_s"], 5),

@@ -68,7 +68,9 @@ struct BBWorkspace {
   size_t n;                 /* Gcc.s3 */
   int nblk;                 /* reduction partial slots */
   double *x, *r, *rs, *p, *v, *t, *b;   /* FP64 Krylov vectors */
+  double *p2, *v2;          /* second buffers of p and v (ping-pong) */
   float *xf, *rf, *rsf, *pf, *vf, *tf, *bf;  /* FP32 inner-solver vectors */
+  float *pf2, *vf2;
   uint8_t *flags;
   double *partials;
   KrylovScalars *sc;        /* device */

@@ -162,6 +162,8 @@ void bb_ensure_workspace(void)
   ws->rs = (double *) dmalloc(n * sizeof(double));
   ws->p = (double *) dmalloc(n * sizeof(double));
   ws->v = (double *) dmalloc(n * sizeof(double));
+  ws->p2 = (double *) dmalloc(n * sizeof(double));
+  ws->v2 = (double *) dmalloc(n * sizeof(double));
   ws->t = (double *) dmalloc(n * sizeof(double));
   ws->b = (double *) dmalloc(n * sizeof(double));
   ws->xf = (float *) dmalloc(n * sizeof(float));
@@ -169,6 +171,8 @@ void bb_ensure_workspace(void)
   ws->rsf = (float *) dmalloc(n * sizeof(float));
   ws->pf = (float *) dmalloc(n * sizeof(float));
   ws->vf = (float *) dmalloc(n * sizeof(float));
+  ws->pf2 = (float *) dmalloc(n * sizeof(float));
+  ws->vf2 = (float *) dmalloc(n * sizeof(float));
   ws->tf = (float *) dmalloc(n * sizeof(float));
   ws->bf = (float *) dmalloc(n * sizeof(float));
   ws->flags = (uint8_t *) dmalloc(n);
@@ -186,7 +190,7 @@ void bb_free_workspace(void)
   BBWorkspace *ws = &g_bb.ws;
   void *ptrs[] = {ws->x, ws->r, ws->rs, ws->p, ws->v, ws->t, ws->b, ws->xf,
     ws->rf, ws->rsf, ws->pf, ws->vf, ws->tf, ws->bf, ws->flags, ws->partials, ws->sc,
-    ws->red};
+    ws->red, ws->p2, ws->v2, ws->pf2, ws->vf2};
   for(size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);
   if(ws->sc_host) BB_CHECK(hipHostFree(ws->sc_host));
   if(ws->red_host) BB_CHECK(hipHostFree(ws->red_host));

@@ -412,6 +412,35 @@ struct OperandP {
   }
 };
 
+/* K1's operand with K4 folded in: p = r + beta p - beta omega v rounded to
+ * T (the reference's update of p), then M p.  s carries the new p. */
+template<typename T>
+struct OperandPNew {
+  struct Raw { T r, p, v; uint8_t f; };
+  const T *__restrict__ r;
+  const T *__restrict__ p;
+  const T *__restrict__ v;
+  double beta, bo;
+  __device__ inline Raw load(const uint8_t *__restrict__ fl, long C) const
+  {
+    Raw w;
+    w.r = r[C];
+    w.p = p[C];
+    w.v = v[C];
+    w.f = fl[C];
+    return w;
+  }
+  __device__ inline CellVal val(const Raw &w, const double *dinv) const
+  {
+    CellVal c;
+    c.f = w.f;
+    c.s = (double) (T) (1. * (double) w.r + beta * (double) w.p +
+      bo * (double) w.v);
+    c.m = dinv[w.f & 127] * c.s;
+    return c;
+  }
+};
+
 /* s = r - alpha v rounded to T, as the reference's vector operation */
 template<typename T>
 struct OperandS {
@@ -618,13 +647,30 @@ struct OutS {
   }
 };
 
-/* K1: v = A M p, partial <r*, v> */
+template<typename T>
+struct OutPNew {
+  const T *__restrict__ rs;
+  T *__restrict__ p;
+  T *__restrict__ v;
+  double acc;
+  __device__ inline void operator()(long C, double y, double pn) {
+    const T vt = (T) y;
+    p[C] = (T) pn;
+    v[C] = vt;
+    acc += (double) rs[C] * (double) vt;
+  }
+};
+
+/* K4 + K1: p' = r + beta (p - omega v), v' = A M p', partial <r*, v'>.
+ * p and v are read from one buffer pair and written to the other: a block
+ * reads its halo cells' old p and v, which neighbouring blocks replace. */
 template<typename T>
 __global__ void __launch_bounds__(SX * SY)
 k_spmv_p(PPGeom G, DinvTable tab, const KrylovScalars *sc,
-  const T *__restrict__ p, const T *__restrict__ rs,
-  const uint8_t *__restrict__ fl, T *__restrict__ v,
-  double *__restrict__ part)
+  const T *__restrict__ r, const T *__restrict__ p_old,
+  const T *__restrict__ v_old, const T *__restrict__ rs,
+  const uint8_t *__restrict__ fl, T *__restrict__ p_new,
+  T *__restrict__ v_new, double *__restrict__ part)
 {
   if(sc->done) return;
   __shared__ double sh[SX * SY / 64];
@@ -633,8 +679,8 @@ k_spmv_p(PPGeom G, DinvTable tab, const KrylovScalars *sc,
   load_dinv(tab, dinv);
   const int lb = xcd_block(G.snblk);
   const SweepTile t = sweep_tile(G, lb);
-  OperandP<T> op{p};
-  OutP<T> out{rs, v, 0.};
+  OperandPNew<T> op{r, p_old, v_old, sc->beta, -sc->beta * sc->omega};
+  OutPNew<T> out{rs, p_new, v_new, 0.};
   sweep_z(G, t, fl, dinv, pl, op, out);
   const double a = sweep_block_sum(out.acc, sh);
   if(threadIdx.x == 0 && threadIdx.y == 0) part[lb] = a;
@@ -708,19 +754,6 @@ k_update_xr(PPGeom G, DinvTable tab, const KrylovScalars *sc,
   double b0 = bb_block_sum(a0, sh);
   double b1 = bb_block_sum(a1, sh);
   if(tid == 0) { part[2*lb] = b0; part[2*lb+1] = b1; }
-}
-
-/* K4: p = r + beta p - beta omega v */
-template<typename T>
-__global__ void __launch_bounds__(256)
-k_update_p_vec(size_t n, const KrylovScalars *sc, T *__restrict__ p,
-  const T *__restrict__ r, const T *__restrict__ v)
-{
-  if(sc->done) return;
-  const double beta = sc->beta, bo = -sc->beta * sc->omega;
-  size_t stride = (size_t) gridDim.x * blockDim.x;
-  for(size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
-    p[q] = (T) (1. * (double) r[q] + beta * (double) p[q] + bo * (double) v[q]);
 }
 
 /* ---------------------------------------------------------------------
@@ -843,7 +876,7 @@ static int elementwise_blocks(size_t n)
  * normalised when `scale_inputs` is 0).  Returns the iteration count. */
 template<typename T>
 static void bicgstab_run(const PPGeom &G, const DinvTable &tab, T *b, T *x,
-  T *r, T *rs, T *p, T *v, T *t, double rel_tol, int max_iter,
+  T *r, T *rs, T *p, T *v, T *p2, T *v2, T *t, double rel_tol, int max_iter,
   int scale_inputs, int poll, int kinds[4])
 {
   BBWorkspace *ws = &g_bb.ws;
@@ -866,32 +899,33 @@ static void bicgstab_run(const PPGeom &G, const DinvTable &tab, T *b, T *x,
   }
   k_f_init<<<1, 256, 0, st>>>(G.nblk, ws->partials, ws->sc, rel_tol, max_iter);
   BB_CHECK_LAUNCH();
+  /* p and v alternate between two buffers; k_init left p = r, v = 0 and
+   * beta = omega = 0, so the first sweep forms p = r */
+  T *pa = p, *pb = p2, *va = v, *vb = v2;
   for(int it = 0; it < max_iter + 1; it++) {
     bb_prof_begin(kinds[0]);
-    k_spmv_p<T><<<G.snblk, sblk, 0, st>>>(G, tab, ws->sc, p, rs, ws->flags,
-      v, ws->partials);
+    k_spmv_p<T><<<G.snblk, sblk, 0, st>>>(G, tab, ws->sc, r, pa, va, rs,
+      ws->flags, pb, vb, ws->partials);
     BB_CHECK_LAUNCH();
     bb_prof_end(kinds[0]);
     k_f1<<<1, 256, 0, st>>>(G.snblk, ws->partials, ws->sc);
     BB_CHECK_LAUNCH();
     bb_prof_begin(kinds[1]);
-    k_spmv_s<T><<<G.snblk, sblk, 0, st>>>(G, tab, ws->sc, r, v, ws->flags,
+    k_spmv_s<T><<<G.snblk, sblk, 0, st>>>(G, tab, ws->sc, r, vb, ws->flags,
       t, ws->partials);
     BB_CHECK_LAUNCH();
     bb_prof_end(kinds[1]);
     k_f2<<<1, 256, 0, st>>>(G.snblk, ws->partials, ws->sc);
     BB_CHECK_LAUNCH();
     bb_prof_begin(kinds[2]);
-    k_update_xr<T><<<G.nblk, blk, 0, st>>>(G, tab, ws->sc, x, p, r, v, t, rs,
-      ws->flags, ws->partials);
+    k_update_xr<T><<<G.nblk, blk, 0, st>>>(G, tab, ws->sc, x, pb, r, vb, t,
+      rs, ws->flags, ws->partials);
     BB_CHECK_LAUNCH();
     bb_prof_end(kinds[2]);
     k_f3<<<1, 256, 0, st>>>(G.nblk, ws->partials, ws->sc);
     BB_CHECK_LAUNCH();
-    bb_prof_begin(kinds[3]);
-    k_update_p_vec<T><<<eb, 256, 0, st>>>(n, ws->sc, p, r, v);
-    BB_CHECK_LAUNCH();
-    bb_prof_end(kinds[3]);
+    T *tp = pa; pa = pb; pb = tp;
+    T *tv = va; va = vb; vb = tv;
     if((it + 1) % poll == 0 || it == max_iter) {
       BB_CHECK(hipMemcpyAsync(ws->sc_host, ws->sc, sizeof(KrylovScalars),
         hipMemcpyDeviceToHost, st));
@@ -904,7 +938,7 @@ static void bicgstab_run(const PPGeom &G, const DinvTable &tab, T *b, T *x,
   BB_CHECK(hipStreamSynchronize(st));
 }
 
-static int g_poll = 4;
+static int g_poll = 8;
 
 extern "C" void bb_set_poll_interval(int n) { g_poll = n > 0 ? n : 1; }
 
@@ -913,7 +947,7 @@ static void solve_fp64(const PPGeom &G, const DinvTable &tab)
   BBWorkspace *ws = &g_bb.ws;
   int kinds[4] = {BB_K_SPMV_P, BB_K_SPMV_S, BB_K_UPDATE_XR, BB_K_UPDATE_P};
   bicgstab_run<double>(G, tab, _rhs_p[0], ws->x, ws->r, ws->rs, ws->p, ws->v,
-    ws->t, pp_residual, pp_max_iter, 1, g_poll, kinds);
+    ws->p2, ws->v2, ws->t, pp_residual, pp_max_iter, 1, g_poll, kinds);
   KrylovScalars *h = ws->sc_host;
   g_bb.stats.iterations = h->iter;
   g_bb.stats.outer_steps = 1;
@@ -924,7 +958,7 @@ static void solve_fp64(const PPGeom &G, const DinvTable &tab)
 /* FP64 refinement around FP32 BiCGSTAB.  Inner solves run to a relative
  * reduction of `inner_tol` (or `inner_max` iterations); each outer step
  * recomputes the true residual in FP64. */
-static double g_inner_tol = 1e-4;
+static double g_inner_tol = 1e-5;
 static int g_inner_max = 2000;
 static int g_outer_max = 50;
 
@@ -988,7 +1022,7 @@ static void solve_mixed(const PPGeom &G, const DinvTable &tab)
     int imax = g_inner_max;
     if(imax > pp_max_iter - total_inner) imax = pp_max_iter - total_inner;
     bicgstab_run<float>(G, tab, ws->bf, ws->xf, ws->rf, ws->rsf, ws->pf, ws->vf,
-      ws->tf, inner_tol, imax, 0, g_poll, kinds);
+      ws->pf2, ws->vf2, ws->tf, inner_tol, imax, 0, g_poll, kinds);
     total_inner += ws->sc_host->iter + (ws->sc_host->s_conv ? 1 : 0);
     bb_prof_begin(BB_K_CORRECT64);
     k_correct64<<<eb, 256, 0, st>>>(n, rn, ws->xf, ws->x);
