@@ -411,6 +411,31 @@ void bb_prof_reset(void);
  * traffic counters (one read and one write of `bytes` per launch). */
 real bb_copy_bandwidth(long bytes, int reps);
 
+/* ---------------------------------------------------------------------
+ * Outside this library's path.  Exported so that the reference driver
+ * links unchanged; each prints which path it belongs to and exits with
+ * EXIT_FAILURE.  Precursor inflow (src/bluebottle.h:2411, 2447, 2877,
+ * 2926), the -DTEST checks (2616, 2628, 2753, 2766, 2780, 2792) and
+ * helpers the step does not call (2704, 2717, 2943, 3341, 3355-3391).
+ * ------------------------------------------------------------------- */
+void cuda_dom_turb_planes_push(int *bc_configs);
+void cuda_dom_turb_planes_pull(int *bc_configs);
+void cuda_compute_turb_forcing(void);
+void cuda_yank_turb_planes(int *bc_flow_configs, real *bc_plane_pos, real *vel);
+void cuda_U_star_test_exp(void);
+void cuda_U_star_test_cos(void);
+void cuda_project_test(void);
+void cuda_BC_test(void);
+void cuda_quad_interp_test(void);
+void cuda_lamb_test(void);
+void cuda_PP_bicgstab_jacobi(void);
+void cuda_div_U(void);
+void cuda_collisions(void);
+real cuda_compute_energy(void);
+void cuda_colocate_Gfx(real *_u, real *_u_co, dom_struct *_dom);
+void cuda_colocate_Gfy(real *_v, real *_v_co, dom_struct *_dom);
+void cuda_colocate_Gfz(real *_w, real *_w_co, dom_struct *_dom);
+
 #ifdef __cplusplus
 }
 #endif
