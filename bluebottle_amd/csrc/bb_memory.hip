@@ -572,3 +572,29 @@ extern "C" real bb_copy_bandwidth(long bytes, int reps)
   gpumem -= 2 * (long) n * 16;
   return ms > 0.f ? 2. * (double) n * 16. * reps / (ms * 1e-3) / 1e9 : 0.;
 }
+
+/* ---- device buffers for callers of the entry points that take device
+ * pointers (cuda_quad_check_nodes, cuda_quad_interp); no counterpart in
+ * the reference, whose callers use cudaMalloc/cudaMemcpy ---- */
+extern "C" void *bb_device_alloc(size_t bytes)
+{
+  bb_ensure_context();
+  return dmalloc(bytes);
+}
+
+extern "C" void bb_device_free(void *ptr)
+{
+  if(!g_bb.ready || !ptr) return;
+  BB_CHECK(hipStreamSynchronize(g_bb.stream));
+  dfree(ptr);
+}
+
+extern "C" void bb_copy_to_device(void *dev, const void *host, size_t bytes)
+{
+  bb_h2d(dev, host, bytes);
+}
+
+extern "C" void bb_copy_to_host(void *host, const void *dev, size_t bytes)
+{
+  bb_d2h(host, dev, bytes);
+}

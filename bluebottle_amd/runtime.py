@@ -43,9 +43,11 @@ ENTRY_POINTS = {
     "cuda_part_p_fill": (None, []), "cuda_store_coeffs": (None, []),
     "cuda_move_parts_sub": (None, []), "cuda_move_parts": (None, []),
     "cuda_parts_internal": (None, []),
-    "cuda_quad_check_nodes": (None, [C.c_int, _PD, _PD, C.c_int]),
-    "cuda_quad_interp": (None, [C.c_int, _PD, _PD, C.c_int, _PD, _PD, _PD,
-                                _PD]),
+    # node tables and node values are device pointers
+    "cuda_quad_check_nodes": (None, [C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
+    "cuda_quad_interp": (None, [C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_void_p]),
     "cuda_Lamb": (None, []), "cuda_lamb_err": (C.c_double, []),
     "bb_set_solver_mode": (None, [C.c_int]),
     "bb_get_solver_mode": (C.c_int, []),
@@ -77,6 +79,10 @@ EXTRA_HIP = {
     "bb_set_implicit": (None, [C.c_int]),
     "bb_is_implicit": (C.c_int, []),
     "bb_copy_bandwidth": (C.c_double, [C.c_long, C.c_int]),
+    "bb_device_alloc": (C.c_void_p, [C.c_size_t]),
+    "bb_device_free": (None, [C.c_void_p]),
+    "bb_copy_to_device": (None, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "bb_copy_to_host": (None, [C.c_void_p, C.c_void_p, C.c_size_t]),
 }
 
 # kernel kinds of the event timer (bb_device.h)

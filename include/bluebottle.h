@@ -21,6 +21,8 @@
 #ifndef BLUEBOTTLE_HIP_H
 #define BLUEBOTTLE_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -410,6 +412,15 @@ void bb_prof_reset(void);
  * reference point for the kernels' HBM rates and for calibrating hardware
  * traffic counters (one read and one write of `bytes` per launch). */
 real bb_copy_bandwidth(long bytes, int reps);
+
+/* Device buffers for the entry points that take device pointers
+ * (cuda_quad_check_nodes, cuda_quad_interp), allocated on the library's
+ * device; the copies go through the library's pinned staging buffer on its
+ * stream and return when done. */
+void *bb_device_alloc(size_t bytes);
+void bb_device_free(void *ptr);
+void bb_copy_to_device(void *dev, const void *host, size_t bytes);
+void bb_copy_to_host(void *host, const void *dev, size_t bytes);
 
 /* ---------------------------------------------------------------------
  * Outside this library's path.  Exported so that the reference driver

@@ -144,6 +144,11 @@ void orc_Lamb(orc_state *s);
 /* src/cuda_quadrature.cu:316-398 + src/quadrature_kernel.cu:578-654 with
  * the cross-particle race removed (per-particle scratch). */
 real orc_lamb_err(orc_state *s);
+/* cuda_quad_check_nodes and cuda_quad_interp on the node tables below:
+ * node flags into parts[].nodes; node values nparts x 26, node fastest. */
+void orc_quad_check_nodes(orc_state *s, const real *theta, const real *phi);
+void orc_quad_interp(orc_state *s, const real *theta, const real *phi,
+  real *pp, real *ur, real *ut, real *up);
 /* Node tables of the 26-point Lebedev rule (src/cuda_quadrature.cu:86-157). */
 void orc_lebedev_nodes(real theta[26], real phi[26], real weight[26]);
 /* Helpers exposed for closed-form tests. */

@@ -321,6 +321,22 @@ static void calc_forces(orc_state *s)
   }
 }
 
+/* cuda_quad_check_nodes (src/cuda_quadrature.cu:30-62,
+ * src/quadrature_kernel.cu:25-93): node flags of every particle. */
+void orc_quad_check_nodes(orc_state *s, const real *theta, const real *phi)
+{
+  check_nodes(s, theta, phi);
+}
+
+/* cuda_quad_interp (src/cuda_quadrature.cu:30-62,
+ * src/quadrature_kernel.cu:95-303): pressure and spherical velocity
+ * components at every node, nparts x 26, node fastest. */
+void orc_quad_interp(orc_state *s, const real *theta, const real *phi,
+  real *pp, real *ur, real *ut, real *up)
+{
+  interpolate_nodes(s, theta, phi, pp, ur, ut, up);
+}
+
 void orc_Lamb(orc_state *s)
 {
   if(s->nparts <= 0) return;

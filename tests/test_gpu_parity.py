@@ -185,6 +185,117 @@ def test_particle_fields(warmed, which):
                    TOL_STENCIL)
 
 
+def _lebedev():
+    from oracle import pyoracle
+    lib = pyoracle.load()
+    th = np.zeros(26)
+    ph = np.zeros(26)
+    wt = np.zeros(26)
+    P = C.POINTER(C.c_double)
+    lib.orc_lebedev_nodes(th.ctypes.data_as(P), ph.ctypes.data_as(P),
+                          wt.ctypes.data_as(P))
+    return th, ph
+
+
+class _DeviceArrays:
+    """Device copies of host arrays through the library's buffer helpers."""
+
+    def __init__(self, hip):
+        self.hip = hip
+        self.ptrs = []
+
+    def put(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = self.hip.bb_device_alloc(a.nbytes)
+        self.hip.bb_copy_to_device(d, a.ctypes.data, a.nbytes)
+        self.ptrs.append(d)
+        return d
+
+    def zeros(self, n):
+        return self.put(np.zeros(n))
+
+    def get(self, d, n):
+        out = np.empty(n)
+        self.hip.bb_copy_to_host(out.ctypes.data, d, out.nbytes)
+        return out
+
+    def free(self):
+        for d in self.ptrs:
+            self.hip.bb_device_free(d)
+        self.ptrs = []
+
+
+def test_quad_check_nodes(warmed):
+    """cuda_quad_check_nodes (src/particle.h:1097) on the device node
+    tables: node flags bit-exact."""
+    name, s = warmed
+    if not s.case.particles:
+        pytest.skip("no particles")
+    th, ph = _lebedev()
+    dev = _DeviceArrays(s.hip)
+    try:
+        o = _resync(s)
+        s.hip.cuda_quad_check_nodes(0, dev.put(th), dev.put(ph), 26)
+        P = C.POINTER(C.c_double)
+        o.lib.orc_quad_check_nodes.argtypes = [C.c_void_p, P, P]
+        o.lib.orc_quad_check_nodes(o.ptr, th.ctypes.data_as(P),
+                                   ph.ctypes.data_as(P))
+        s.pull()
+        a = [list(p.nodes) for p in s.parts()]
+        b = [list(o.s.parts[i].nodes) for i in range(len(a))]
+        assert a == b
+    finally:
+        dev.free()
+
+
+def test_quad_interp(warmed):
+    """cuda_quad_interp (src/particle.h:1114): pressure and spherical
+    velocity components at every node, 1e-12 relative."""
+    name, s = warmed
+    if not s.case.particles:
+        pytest.skip("no particles")
+    th, ph = _lebedev()
+    n = 26 * len(s.case.particles)
+    dev = _DeviceArrays(s.hip)
+    try:
+        s.hip.cuda_quad_check_nodes(0, dev.put(th), dev.put(ph), 26)
+        o = _resync(s)
+        outs = [dev.zeros(n) for _ in range(4)]
+        s.hip.cuda_quad_interp(0, dev.ptrs[0], dev.ptrs[1], 26, *outs)
+        gpu = [dev.get(d, n) for d in outs]
+        P = C.POINTER(C.c_double)
+        ref = [np.zeros(n) for _ in range(4)]
+        o.lib.orc_quad_check_nodes.argtypes = [C.c_void_p, P, P]
+        o.lib.orc_quad_interp.argtypes = [C.c_void_p, P, P, P, P, P, P]
+        o.lib.orc_quad_check_nodes(o.ptr, th.ctypes.data_as(P),
+                                   ph.ctypes.data_as(P))
+        o.lib.orc_quad_interp(o.ptr, th.ctypes.data_as(P), ph.ctypes.data_as(P),
+                              *[r.ctypes.data_as(P) for r in ref])
+        for label, a, b in zip(("pp", "ur", "ut", "up"), gpu, ref):
+            e = rel_err(a, b)
+            assert e <= TOL_STENCIL, "%s rel err %.3e" % (label, e)
+    finally:
+        dev.free()
+
+
+def test_part_bc_p(warmed):
+    """cuda_part_BC_p (src/bluebottle.h:2538): the Poisson rhs masked to
+    fluid cells off the particle shells, bit-exact on seeded values."""
+    name, s = warmed
+    o = _resync(s)
+    nrhs = s.dom.Gcc.s3
+    vals = np.random.default_rng(11).standard_normal(nrhs)
+    rhs_dev = C.POINTER(C.POINTER(C.c_double)).in_dll(s.host, "_rhs_p")[0]
+    addr = C.cast(rhs_dev, C.c_void_p).value
+    s.hip.bb_copy_to_device(addr, vals.ctypes.data, vals.nbytes)
+    o.array("rhs_p")[:] = vals
+    s.hip.cuda_part_BC_p(0)
+    o.call("part_BC_p")
+    got = np.empty(nrhs)
+    s.hip.bb_copy_to_host(got.ctypes.data, addr, got.nbytes)
+    assert np.array_equal(got, o.array("rhs_p")[:nrhs])
+
+
 def test_lamb(warmed):
     name, s = warmed
     if not s.case.particles:
