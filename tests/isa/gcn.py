@@ -98,6 +98,7 @@ class Kernel:
         self.private_size = 0
         self.wg_id = (1, 0, 0)
         self.user_sgprs = 2
+        self.dispatch_ptr = False
 
 
 _INST = re.compile(r"^\s+([a-z_0-9]+)(.*?)(?:\s*//\s*([0-9A-F]+):.*)?$")
@@ -163,11 +164,15 @@ def load_code_object(path, objdump="/opt/rocm/llvm/bin/llvm-objdump",
         k.wg_id = (field("system_sgpr_workgroup_id_x"),
                    field("system_sgpr_workgroup_id_y"),
                    field("system_sgpr_workgroup_id_z"))
-        for f in ("user_sgpr_dispatch_ptr", "user_sgpr_queue_ptr",
-                  "user_sgpr_dispatch_id", "user_sgpr_private_segment_size"):
+        for f in ("user_sgpr_queue_ptr", "user_sgpr_dispatch_id",
+                  "user_sgpr_private_segment_size"):
             if field(f):
                 raise NotImplementedError("kernel %s uses %s" % (name, f))
-        k.user_sgprs = 2 if field("user_sgpr_kernarg_segment_ptr") else 0
+        # user SGPRs in hardware order: dispatch packet pointer, then the
+        # kernarg segment pointer
+        k.dispatch_ptr = bool(field("user_sgpr_dispatch_ptr"))
+        k.user_sgprs = (2 if k.dispatch_ptr else 0) + \
+            (2 if field("user_sgpr_kernarg_segment_ptr") else 0)
     return kernels
 
 
@@ -221,6 +226,7 @@ _REG = re.compile(r"^([vsa])(?:\[(\d+):(\d+)\]|(\d+))$")
 
 
 POISON = 0xBAD0BAD1
+DISPATCH_BASE = 0x800
 
 
 def _two_over_pi_bits(nbits):
@@ -480,6 +486,16 @@ class Exec:
     # ---- launch ----
     def run(self):
         k = self.k
+        if k.dispatch_ptr and not self.mem.find(DISPATCH_BASE, 64)[0]:
+            # an hsa_kernel_dispatch_packet_t with the launch shape
+            pk = bytearray(64)
+            bx, by, bz = self.block
+            gx, gy, gz = self.grid
+            pk[4:10] = struct.pack("<HHH", bx, by, bz)
+            pk[12:24] = struct.pack("<III", gx * bx, gy * by, gz * bz)
+            pk[28:32] = struct.pack("<I", k.lds_size)
+            pk[40:48] = struct.pack("<Q", self.ka)
+            self.mem.add(DISPATCH_BASE, pk)
         bx, by, bz = self.block
         nth = bx * by * bz
         nwaves = (nth + 63) // 64
@@ -504,9 +520,15 @@ class Exec:
             ty = (t // bx) % by
             tz = t // (bx * by)
             w.v[0] = (tx | (ty << 10) | (tz << 20)).astype(np.uint32)
-            w.s[0] = self.ka & M32
-            w.s[1] = (self.ka >> 32) & M32
-            w.sdef[0] = w.sdef[1] = True
+            ka_reg = 0
+            if k.dispatch_ptr:
+                w.s[0] = DISPATCH_BASE & M32
+                w.s[1] = (DISPATCH_BASE >> 32) & M32
+                w.sdef[0] = w.sdef[1] = True
+                ka_reg = 2
+            w.s[ka_reg] = self.ka & M32
+            w.s[ka_reg + 1] = (self.ka >> 32) & M32
+            w.sdef[ka_reg] = w.sdef[ka_reg + 1] = True
             r = k.user_sgprs
             for d in range(3):
                 if k.wg_id[d]:
