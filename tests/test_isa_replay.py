@@ -1,0 +1,55 @@
+"""The gfx950 machine code of the ghost-cell BC kernel, replayed on the CPU
+(tests/isa) over every launch of one wall-bounded step captured from the
+host emulation: no access outside an allocation, no load based on an
+unwritten scalar register, results equal to the emulation's.
+
+An earlier k_bc_axis, which selected kernel-argument fields by a run-time
+axis index, compiled to code that loaded an extent through a scalar
+register pair its axis-2 path never wrote; this replay is how that was
+found, and it runs here so that such code cannot return unnoticed.  (The
+full replay of every kernel of the step takes tens of minutes and is run
+by hand: tests/isa/capture.py, tests/isa/replay.py.)"""
+import glob
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EMU = os.path.join(ROOT, "tests", "emu")
+ISA = os.path.join(ROOT, "tests", "isa")
+
+
+def test_bc_kernel_machine_code_replays_clean(tmp_path):
+    from test_emu import _asan
+    from bluebottle_amd import runtime
+    asan = _asan()
+    if not os.path.isabs(asan):
+        pytest.skip("AddressSanitizer runtime not available")
+    assert runtime.libs_present(), "build the library first"
+    subprocess.run(["make", "-s", "-j8", "-C", EMU], check=True)
+    cap = tmp_path / "cap"
+    env = dict(os.environ)
+    env.update(LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0",
+               BB_EMU_ISA_KERNELS="k_bc_axis", BB_EMU_ISA_MAX="60")
+    r = subprocess.run([sys.executable, os.path.join(ISA, "capture.py"),
+                        "channel_free", str(cap), "1"], env=env,
+                       capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    # set-up and one step: dom_BC twice, BC_star twice, BC_phi, dom_BC,
+    # BC_p and dom_BC, three axis launches each
+    launches = glob.glob(str(cap / "*.json"))
+    assert len(launches) == 24, len(launches)
+    assert all(json.load(open(f))["kernel"] == "k_bc_axis" for f in launches)
+    env = dict(os.environ)
+    env["BB_ISA_OBJDIR"] = str(tmp_path / "co")
+    sys.path.insert(0, ISA)
+    import replay
+    replay.extract_code_objects(str(tmp_path / "co"))
+    r = subprocess.run([sys.executable, os.path.join(ISA, "replay.py"),
+                        str(cap)], env=env, capture_output=True, text=True,
+                       timeout=1200)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    assert "replayed 24 launches, 0 with findings" in r.stdout
