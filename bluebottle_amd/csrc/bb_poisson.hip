@@ -223,25 +223,60 @@ k_pp_setup(PPGeom G, BBGeom B, double rho_dt, int mask,
 }
 
 /* ---------------------------------------------------------------------
- * Finishing kernels (one block of 256): sum the per-block partials in a
- * fixed order and advance the scalar state.
+ * Finishing kernels (one block of up to 1024 threads): sum the per-block
+ * partials (NQ quantities per block, interleaved) in a fixed order and
+ * advance the scalar state.  Each thread walks its blocks b = tid,
+ * tid + blockDim, ... in order; the loads of four of them are issued
+ * before their adds, so the walk costs one memory latency per four
+ * blocks instead of one per block.
  * ------------------------------------------------------------------- */
-__device__ inline void sum_partials(const double *part, int nblk, int nq,
-  double *out)
+template<int NQ>
+__device__ inline void sum_partials(const double *part, int nblk, double *out)
 {
-  __shared__ double sh[4];
-  for(int q = 0; q < nq; q++) {
-    double a = 0.;
-    for(int b = threadIdx.x; b < nblk; b += blockDim.x) a += part[b*nq + q];
-    out[q] = bb_block_sum(a, sh);
+  __shared__ double sh[NQ][16];
+  const int tid = threadIdx.x, nth = blockDim.x;
+  double a[NQ];
+#pragma unroll
+  for(int q = 0; q < NQ; q++) a[q] = 0.;
+  int b = tid;
+  for(; b + 3 * nth < nblk; b += 4 * nth) {
+    double v[4][NQ];
+#pragma unroll
+    for(int u = 0; u < 4; u++)
+#pragma unroll
+      for(int q = 0; q < NQ; q++) v[u][q] = part[(b + u * nth) * NQ + q];
+#pragma unroll
+    for(int u = 0; u < 4; u++)
+#pragma unroll
+      for(int q = 0; q < NQ; q++) a[q] += v[u][q];
+  }
+  for(; b < nblk; b += nth)
+#pragma unroll
+    for(int q = 0; q < NQ; q++) a[q] += part[b * NQ + q];
+#pragma unroll
+  for(int q = 0; q < NQ; q++) {
+    double w = bb_wave_sum(a[q]);
+    if((tid & 63) == 0) sh[q][tid >> 6] = w;
+  }
+  __syncthreads();
+  if(tid == 0) {
+#pragma unroll
+    for(int q = 0; q < NQ; q++) {
+      double t = 0.;
+      for(int w = 0; w < (nth + 63) >> 6; w++) t += sh[q][w];
+      out[q] = t;
+    }
   }
 }
+
+/* threads of the per-iteration finishing kernels */
+#define FIN_THREADS 1024
 
 /* norm of the raw rhs; 1 when it is 0 (src/cuda_bicgstab.cu:996-1000) */
 __global__ void k_f_norm(int nblk, const double *part, KrylovScalars *sc)
 {
   double o[1];
-  sum_partials(part, nblk, 1, o);
+  sum_partials<1>(part, nblk, o);
   if(threadIdx.x == 0) {
     double nrm = sqrt(o[0]);
     if(nrm == 0) nrm = 1.;
@@ -254,7 +289,7 @@ __global__ void k_f_init(int nblk, const double *part, KrylovScalars *sc,
   double rel_tol, int max_iter)
 {
   double o[2];
-  sum_partials(part, nblk, 2, o);
+  sum_partials<2>(part, nblk, o);
   if(threadIdx.x == 0) {
     sc->tol = 0. + rel_tol * sqrt(o[0]);
     sc->rho = o[1];
@@ -272,7 +307,7 @@ __global__ void k_f1(int nblk, const double *part, KrylovScalars *sc)
 {
   if(sc->done) return;
   double o[1];
-  sum_partials(part, nblk, 1, o);
+  sum_partials<1>(part, nblk, o);
   if(threadIdx.x == 0) sc->alpha = sc->rho / o[0];
 }
 
@@ -280,7 +315,7 @@ __global__ void k_f2(int nblk, const double *part, KrylovScalars *sc)
 {
   if(sc->done) return;
   double o[3];
-  sum_partials(part, nblk, 3, o);
+  sum_partials<3>(part, nblk, o);
   if(threadIdx.x == 0) {
     double rs = sqrt(o[2]);
     if(rs <= sc->tol) {   /* monitor.finished(s) */
@@ -296,7 +331,7 @@ __global__ void k_f3(int nblk, const double *part, KrylovScalars *sc)
 {
   if(sc->done) return;
   double o[2];
-  sum_partials(part, nblk, 2, o);
+  sum_partials<2>(part, nblk, o);
   if(threadIdx.x == 0) {
     if(sc->s_conv) { sc->done = 1; return; }
     double rho_new = o[0];
@@ -779,7 +814,7 @@ k_unscale_sum(size_t n, const KrylovScalars *sc, double *__restrict__ x,
 __global__ void k_f_mean(int nblk, const double *part, double *out, double n)
 {
   double o[1];
-  sum_partials(part, nblk, 1, o);
+  sum_partials<1>(part, nblk, o);
   if(threadIdx.x == 0) out[0] = o[0] / n;
 }
 
@@ -857,7 +892,7 @@ k_correct64(size_t n, const double *__restrict__ scal,
 __global__ void k_f_rnorm(int nblk, const double *part, double *out)
 {
   double o[1];
-  sum_partials(part, nblk, 1, o);
+  sum_partials<1>(part, nblk, o);
   if(threadIdx.x == 0) out[0] = sqrt(o[0]);
 }
 
@@ -897,7 +932,7 @@ static void bicgstab_run(const PPGeom &G, const DinvTable &tab, T *b, T *x,
     k_scale2<T><<<eb, 256, 0, st>>>(n, ws->sc, x, b);
     BB_CHECK_LAUNCH();
   }
-  k_f_init<<<1, 256, 0, st>>>(G.nblk, ws->partials, ws->sc, rel_tol, max_iter);
+  k_f_init<<<1, FIN_THREADS, 0, st>>>(G.nblk, ws->partials, ws->sc, rel_tol, max_iter);
   BB_CHECK_LAUNCH();
   /* p and v alternate between two buffers; k_init left p = r, v = 0 and
    * beta = omega = 0, so the first sweep forms p = r */
@@ -908,21 +943,21 @@ static void bicgstab_run(const PPGeom &G, const DinvTable &tab, T *b, T *x,
       ws->flags, pb, vb, ws->partials);
     BB_CHECK_LAUNCH();
     bb_prof_end(kinds[0]);
-    k_f1<<<1, 256, 0, st>>>(G.snblk, ws->partials, ws->sc);
+    k_f1<<<1, FIN_THREADS, 0, st>>>(G.snblk, ws->partials, ws->sc);
     BB_CHECK_LAUNCH();
     bb_prof_begin(kinds[1]);
     k_spmv_s<T><<<G.snblk, sblk, 0, st>>>(G, tab, ws->sc, r, vb, ws->flags,
       t, ws->partials);
     BB_CHECK_LAUNCH();
     bb_prof_end(kinds[1]);
-    k_f2<<<1, 256, 0, st>>>(G.snblk, ws->partials, ws->sc);
+    k_f2<<<1, FIN_THREADS, 0, st>>>(G.snblk, ws->partials, ws->sc);
     BB_CHECK_LAUNCH();
     bb_prof_begin(kinds[2]);
     k_update_xr<T><<<G.nblk, blk, 0, st>>>(G, tab, ws->sc, x, pb, r, vb, t,
       rs, ws->flags, ws->partials);
     BB_CHECK_LAUNCH();
     bb_prof_end(kinds[2]);
-    k_f3<<<1, 256, 0, st>>>(G.nblk, ws->partials, ws->sc);
+    k_f3<<<1, FIN_THREADS, 0, st>>>(G.nblk, ws->partials, ws->sc);
     BB_CHECK_LAUNCH();
     T *tp = pa; pa = pb; pb = tp;
     T *tv = va; va = vb; vb = tv;
