@@ -123,6 +123,8 @@ inline float __shfl_down(float v, unsigned d, int w = 64) { return bb_emu::shfl_
 inline int __shfl_down(int v, unsigned d, int w = 64) { return bb_emu::shfl_down_i(v, d, w); }
 
 inline int atomicAdd(int *a, int v) { return __atomic_fetch_add(a, v, __ATOMIC_SEQ_CST); }
+inline int atomicExch(int *a, int v) { return __atomic_exchange_n(a, v, __ATOMIC_SEQ_CST); }
+inline void __threadfence() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 inline unsigned atomicAdd(unsigned *a, unsigned v) { return __atomic_fetch_add(a, v, __ATOMIC_SEQ_CST); }
 inline double atomicAdd(double *a, double v)
 {

@@ -574,7 +574,10 @@ class Exec:
 
     # ---- instruction semantics ----
     def execute(self, w, op, rest, tgt, where):
-        if op in ("s_waitcnt", "s_nop", "s_setprio", "s_sleep"):
+        if op in ("s_waitcnt", "s_nop", "s_setprio", "s_sleep",
+                  # L2 write-back / invalidate of a device-scope fence: the
+                  # replay has one memory and no caches
+                  "buffer_wbl2", "buffer_inv"):
             return
         if op == "s_endpgm":
             w.done = True
@@ -1427,7 +1430,7 @@ class Exec:
             sv_ = v - (1 << 32) if v & (1 << 31) else v
             new = {"add": (old + v) & M32, "smax": max(so, sv_) & M32,
                    "smin": min(so, sv_) & M32, "umax": max(old, v),
-                   "umin": min(old, v)}[aop]
+                   "umin": min(old, v), "swap": v}[aop]
             self.mem.write(int(addr[l]), new.to_bytes(4, "little"), where + (int(l),))
             out[l] = old
         if dst:
