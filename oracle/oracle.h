@@ -75,6 +75,9 @@ void orc_free(orc_state *s);
  * in *n and its element type in *is_int (0 = real, 1 = int). */
 void *orc_array(orc_state *s, const char *name, size_t *n, int *is_int);
 size_t orc_sizeof_state(void);
+/* threads of the row-parallel loops (matrix assembly and products; rows are
+ * independent, so the count changes no result); 0 keeps the default */
+void orc_set_threads(int n);
 
 /* --- field path (oracle_field.c) --- */
 /* src/bluebottle_kernel.cu:1504-2003 via src/cuda_bluebottle.cu:2930. */

@@ -164,8 +164,9 @@ void orc_pp_matvec(const orc_state *s, const real *x, real *y)
   int off[13];
   dia_offsets(&Gc, off);
   int N = Gc.s3;
-  real v[13];
+  #pragma omp parallel for schedule(static)
   for(int C = 0; C < N; C++) {
+    real v[13];
     orc_pp_coeffs(s, C, v);
     real sum = 0.;
     for(int q = 0; q < 13; q++) {
@@ -176,10 +177,56 @@ void orc_pp_matvec(const orc_state *s, const real *x, real *y)
   }
 }
 
+/* The assembled matrix of one solve: the 13 DIA values of every row, as
+ * the reference stores them (src/cuda_bicgstab.cu:735-747), built once. */
+typedef struct { int N; int off[13]; real *vals; } dia_t;
+
+static void dia_build(const orc_state *s, dia_t *A)
+{
+  const grid_info Gc = s->dom.Gcc;
+  A->N = Gc.s3;
+  dia_offsets(&Gc, A->off);
+  A->vals = malloc((size_t) A->N * 13 * sizeof(real));
+  #pragma omp parallel for schedule(static)
+  for(int C = 0; C < A->N; C++) orc_pp_coeffs(s, C, A->vals + (size_t) 13 * C);
+}
+
+/* y = A x with each row summed over the 13 diagonals in order, exactly as
+ * orc_pp_matvec (rows are independent, so the threads do not change a
+ * result) */
+static void dia_apply(const dia_t *A, const real *x, real *y)
+{
+  const int N = A->N;
+  #pragma omp parallel for schedule(static)
+  for(int C = 0; C < N; C++) {
+    const real *v = A->vals + (size_t) 13 * C;
+    real sum = 0.;
+    for(int q = 0; q < 13; q++) {
+      int col = C + A->off[q];
+      if(col >= 0 && col < N) sum += v[q] * x[col];
+    }
+    y[C] = sum;
+  }
+}
+
+/* Dot product in a fixed order independent of the thread count: serial
+ * sums over consecutive chunks of DOT_CHUNK entries, then the chunk sums
+ * in order. */
+#define DOT_CHUNK 4096
 static real dot(const real *a, const real *b, int n)
 {
+  int nc = (n + DOT_CHUNK - 1) / DOT_CHUNK;
+  real *part = malloc((size_t) (nc > 0 ? nc : 1) * sizeof(real));
+  #pragma omp parallel for schedule(static)
+  for(int c = 0; c < nc; c++) {
+    int lo = c * DOT_CHUNK, hi = lo + DOT_CHUNK < n ? lo + DOT_CHUNK : n;
+    real t = 0.;
+    for(int i = lo; i < hi; i++) t += a[i] * b[i];
+    part[c] = t;
+  }
   real s = 0.;
-  for(int i = 0; i < n; i++) s += a[i] * b[i];
+  for(int c = 0; c < nc; c++) s += part[c];
+  free(part);
   return s;
 }
 
@@ -193,15 +240,14 @@ int orc_bicgstab(const orc_state *s, const real *b, real *x, int max_iter,
   real *sv = malloc(N * sizeof(real)), *Mp = malloc(N * sizeof(real));
   real *AMp = malloc(N * sizeof(real)), *Ms = malloc(N * sizeof(real));
   real *AMs = malloc(N * sizeof(real));
-  real v[13];
-  for(int C = 0; C < N; C++) {
-    orc_pp_coeffs(s, C, v);
-    dinv[C] = 1. / v[6];
-  }
+  dia_t A;
+  dia_build(s, &A);
+  for(int C = 0; C < N; C++) dinv[C] = 1. / A.vals[(size_t) 13 * C + 6];
   real tol = 0. + rel_tol * sqrt(dot(b, b, N));
   int iter = 0;
   real resid;
-  orc_pp_matvec(s, x, y);
+  dia_apply(&A, x, y);
+  #pragma omp parallel for schedule(static)
   for(int i = 0; i < N; i++) r[i] = 1. * b[i] + -1. * y[i];
   memcpy(p, r, N * sizeof(real));
   memcpy(rs, r, N * sizeof(real));
@@ -209,23 +255,30 @@ int orc_bicgstab(const orc_state *s, const real *b, real *x, int max_iter,
   for(;;) {
     resid = sqrt(dot(r, r, N));
     if(resid <= tol || iter >= max_iter) break;
+    #pragma omp parallel for schedule(static)
     for(int i = 0; i < N; i++) Mp[i] = dinv[i] * p[i];
-    orc_pp_matvec(s, Mp, AMp);
+    dia_apply(&A, Mp, AMp);
     real alpha = rho_old / dot(rs, AMp, N);
+    #pragma omp parallel for schedule(static)
     for(int i = 0; i < N; i++) sv[i] = 1. * r[i] + -alpha * AMp[i];
     resid = sqrt(dot(sv, sv, N));
     if(resid <= tol || iter >= max_iter) {
+      #pragma omp parallel for schedule(static)
       for(int i = 0; i < N; i++) x[i] = 1. * x[i] + alpha * Mp[i];
       break;
     }
+    #pragma omp parallel for schedule(static)
     for(int i = 0; i < N; i++) Ms[i] = dinv[i] * sv[i];
-    orc_pp_matvec(s, Ms, AMs);
+    dia_apply(&A, Ms, AMs);
     real omega = dot(AMs, sv, N) / dot(AMs, AMs, N);
+    #pragma omp parallel for schedule(static)
     for(int i = 0; i < N; i++) x[i] = 1. * x[i] + alpha * Mp[i] + omega * Ms[i];
+    #pragma omp parallel for schedule(static)
     for(int i = 0; i < N; i++) r[i] = 1. * sv[i] + -omega * AMs[i];
     real rho_new = dot(rs, r, N);
     real beta = (rho_new / rho_old) * (alpha / omega);
     rho_old = rho_new;
+    #pragma omp parallel for schedule(static)
     for(int i = 0; i < N; i++)
       p[i] = 1. * r[i] + beta * p[i] + -beta * omega * AMp[i];
     iter++;
@@ -233,7 +286,7 @@ int orc_bicgstab(const orc_state *s, const real *b, real *x, int max_iter,
   *resid_out = resid;
   *converged = resid <= tol;
   free(dinv); free(y); free(p); free(r); free(rs); free(sv); free(Mp);
-  free(AMp); free(Ms); free(AMs);
+  free(AMp); free(Ms); free(AMs); free(A.vals);
   return iter;
 }
 

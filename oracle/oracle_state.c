@@ -114,3 +114,10 @@ void *orc_array(orc_state *s, const char *name, size_t *n, int *is_int)
   *is_int = 0;
   return NULL;
 }
+
+#ifdef _OPENMP
+#include <omp.h>
+void orc_set_threads(int n) { if(n > 0) omp_set_num_threads(n); }
+#else
+void orc_set_threads(int n) { (void) n; }
+#endif
