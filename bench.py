@@ -67,7 +67,11 @@ def kernel_bytes(n, xn, yn, zn):
     }
 
 
-def run_mode(s, mode, steps, warmup, runtime):
+def run_mode(s, mode, steps, warmup, prof_steps, runtime):
+    """Warm-up steps, then `steps` timed steps with no instrumentation (the
+    rate), then `prof_steps` more steps with HIP events recorded on the
+    library stream around every launch (the per-kernel table and the
+    roofline)."""
     host, hip = s.host, s.hip
     s.set_solver(mode)
     for i in range(warmup):
@@ -80,8 +84,7 @@ def run_mode(s, mode, steps, warmup, runtime):
     tot = C.c_long.in_dll(host, "bb_rec_total_iters")
     calls = C.c_int.in_dll(host, "bb_rec_calls")
     it0, c0 = tot.value, calls.value
-    hip.bb_prof_reset()
-    hip.bb_prof_enable(1)
+    hip.bb_prof_enable(0)
     lamb_iters = 0
     hip.bb_synchronize()
     t0 = time.perf_counter()
@@ -91,6 +94,18 @@ def run_mode(s, mode, steps, warmup, runtime):
             mode, i, time.perf_counter() - t0))
     hip.bb_synchronize()
     elapsed = time.perf_counter() - t0
+    it1, c1 = tot.value, calls.value
+    # profiled steps: per-kernel event timing
+    hip.bb_prof_reset()
+    hip.bb_prof_enable(1)
+    hip.bb_synchronize()
+    t1 = time.perf_counter()
+    pl = 0
+    for i in range(prof_steps):
+        pl += s.step()
+        log("mode %d profiled step %d done" % (mode, i))
+    hip.bb_synchronize()
+    prof_elapsed = time.perf_counter() - t1
     hip.bb_prof_enable(0)
     prof = {}
     for k, name in enumerate(runtime.PROF_KINDS):
@@ -104,9 +119,14 @@ def run_mode(s, mode, steps, warmup, runtime):
         "ms_per_step": 1e3 * elapsed / steps,
         "timesteps_per_s": steps / elapsed,
         "lamb_iters_per_step": lamb_iters / steps,
-        "krylov_iters_per_solve": (tot.value - it0) / max(calls.value - c0, 1),
-        "solves": calls.value - c0,
-        "krylov_iters": tot.value - it0,
+        "krylov_iters_per_solve": (it1 - it0) / max(c1 - c0, 1),
+        "solves": c1 - c0,
+        "krylov_iters": it1 - it0,
+        "profiled_steps": prof_steps,
+        "profiled_ms_per_step": (1e3 * prof_elapsed / prof_steps
+                                 if prof_steps else None),
+        "profiled_krylov_iters_per_solve": (
+            (tot.value - it1) / max(calls.value - c1, 1)),
         "prof": prof,
     }
 
@@ -151,6 +171,17 @@ def measured_traffic(kernel, workload):
     return rec["total"] if rec else None
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(s, fp64, krylov_sample, desc):
     """Serial FP64 oracle on this host, bounded sample, extrapolated."""
     import ctypes as C
@@ -158,6 +189,7 @@ def cpu_baseline(s, fp64, krylov_sample, desc):
     s.pull()
     o = OracleState.from_sim(s)
     lib = o.lib
+    lib.orc_set_threads(1)
     nparts = o.nparts
 
     def timed(fn):
@@ -223,7 +255,8 @@ def cpu_baseline(s, fp64, krylov_sample, desc):
     del o
     return {
         "value": round(1.0 / step_s, 6), "unit": "timesteps/s", "cores": 1,
-        "kind": "port",
+        "kind": "port", "host_nproc": os.cpu_count(),
+        "host_cpu": cpu_model(),
         "sample": ("serial FP64 oracle on the full state of the workload (%s): "
                    "one pass of each sub-iteration stage (%.1f s), the step's "
                    "outer stages (%.1f s) and %d BiCGSTAB iterations "
@@ -248,6 +281,9 @@ def main():
                          "configuration); tg: the same box as a sphere-free "
                          "Taylor-Green flow")
     ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--prof-steps", type=int, default=1,
+                    help="steps after the timed ones with per-launch event "
+                         "timing (the kernel table and the roofline)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--krylov-sample", type=int, default=10)
     ap.add_argument("--mixed-inner-tol", type=float, default=0.0,
@@ -310,7 +346,8 @@ def main():
             # the achievable streaming rate on this device: a plain 1 GiB copy
             copy_gbs = s.hip.bb_copy_bandwidth(1 << 30, 10)
             log("plain HBM copy: %.0f GB/s" % copy_gbs)
-        modes[m] = run_mode(s, names[m], args.steps, args.warmup, runtime)
+        modes[m] = run_mode(s, names[m], args.steps, args.warmup,
+                            args.prof_steps, runtime)
         log("%s: %.4f timesteps/s, %.2f Lamb it/step, %.1f Krylov it/solve" % (
             m, modes[m]["timesteps_per_s"], modes[m]["lamb_iters_per_step"],
             modes[m]["krylov_iters_per_solve"]))

@@ -135,6 +135,23 @@ def small_particle_case(n=24, nparts=3, radius_dx=3.0, seed=7, order=4,
                 gradP=(0.0, 0.0, 0.0, 0.0, -0.05, 0.0))
 
 
+def wide_case(n=(72, 32, 16), nparts=2, seed=9):
+    """A periodic box of cubic cells, longer in x than one tile of the
+    Poisson sweeps (64 cells), with a few spheres of radius 3 dx."""
+    nx, ny, nz = n
+    dx = 1.0 / nx
+    hi = (1.0, ny * dx, nz * dx)
+    a = 3.0 * dx
+    pos = place_spheres(nparts, a, (0, 0, 0), hi, 2 * dx, seed)
+    parts = [Particle(x=float(p[0]), y=float(p[1]), z=float(p[2]), r=a,
+                      rho=2.0) for p in pos]
+    return Case(n=(nx, ny, nz), hi=hi, nu=0.01, CFL=0.3, pp_max_iter=4000,
+                pp_residual=1e-11, lamb_max_iter=2, init="taylor_green",
+                noise=1e-3, seed=seed, particles=parts,
+                g=(0.0, 0.0, 0.0, 0.0, -1.0, 0.0),
+                gradP=(0.0, 0.0, 0.0, 0.0, -0.05, 0.0))
+
+
 def channel_case(n=(16, 12, 16), nparts=0, seed=5):
     """Wall-bounded shear box: no-slip y walls (Dirichlet u = +-U_w), p
     Neumann, periodic x and z."""

@@ -212,7 +212,9 @@ static void dia_apply(const dia_t *A, const real *x, real *y)
 /* Dot product in a fixed order independent of the thread count: serial
  * sums over consecutive chunks of DOT_CHUNK entries, then the chunk sums
  * in order. */
+#ifndef DOT_CHUNK
 #define DOT_CHUNK 4096
+#endif
 static real dot(const real *a, const real *b, int n)
 {
   int nc = (n + DOT_CHUNK - 1) / DOT_CHUNK;

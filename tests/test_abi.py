@@ -136,3 +136,35 @@ def test_outside_path_entry_points_exit_with_a_message():
     assert "cuda_compute_turb_forcing" in p.stderr
     assert "outside this library's path" in p.stderr
     assert "returned" not in p.stdout
+
+
+def _undefined_unversioned(path):
+    out = subprocess.run(["nm", "-D", "-u", path], check=True,
+                         capture_output=True, text=True).stdout
+    names = set()
+    for line in out.splitlines():
+        parts = line.split()
+        if parts and "@" not in parts[-1]:
+            names.add(parts[-1])
+    return names - {"__gmon_start__"}
+
+
+def test_reference_driver_links_against_the_library():
+    """The link of the reference's unchanged C driver (the objects of its
+    Makefile:49-56) against libbluebottle_hip.so, checked symbol by symbol
+    from tests/golden/driver_symbols.json (written from the driver's source
+    text by tests/golden/make_driver_symbols.py): every cuda_* function the
+    driver calls is exported, and every symbol the library leaves for the
+    executable to define is a global or a function the driver defines.  (A
+    compile of the driver itself needs cgnslib.h, which this image lacks.)"""
+    import json
+    with open(os.path.join(ROOT, "tests", "golden",
+                           "driver_symbols.json")) as f:
+        drv = json.load(f)
+    hip = defined_symbols(runtime.HIP_LIB)
+    missing = [c for c in drv["calls"] if c not in hip]
+    assert not missing, "driver calls not exported: %s" % missing
+    provided = set(drv["globals"]) | set(drv["functions"])
+    unresolved = sorted(_undefined_unversioned(runtime.HIP_LIB) - provided)
+    assert not unresolved, "library needs symbols the driver does not " \
+        "define: %s" % unresolved

@@ -16,7 +16,7 @@ import math
 import numpy as np
 import pytest
 
-from bluebottle_amd import configs, sim
+from bluebottle_amd import abi, configs, sim
 from oracle.pyoracle import OracleState
 
 from helpers import (compare_fields, compare_ints, compare_parts, rel_err,
@@ -53,6 +53,9 @@ CASES = {
     "channel_free": lambda: configs.channel_case(n=(16, 12, 16), nparts=0),
     "tg_free": lambda: configs.periodic_case(16, 0.01, CFL=0.3, pp_max_iter=500,
                                              pp_residual=1e-11),
+    # two x tiles of the SpMV sweeps (interior tile halo), tile counts
+    # divisible by 8 (XCD block remap), several blocks along x
+    "wide": lambda: configs.wide_case(),
 }
 
 
@@ -328,6 +331,45 @@ def test_move_parts(warmed, which):
         pytest.skip("no particles")
     o = _stage(s, getattr(s.hip, "cuda_" + which), lambda o: o.call(which))
     compare_parts(s, o, 1e-12)
+
+
+def test_store_coeffs(warmed):
+    """cuda_store_coeffs (src/cuda_particle.cu:1046-1081): the coefficient
+    history shift, bit-exact."""
+    _, s = warmed
+    if not s.case.particles:
+        pytest.skip("no particles")
+    s.hip.cuda_Lamb()
+    o = _stage(s, s.hip.cuda_store_coeffs, lambda o: o.call("store_coeffs"))
+    for c in COEFF_SET:
+        for suf in ("", "0", "00"):
+            assert np.array_equal(s.array(c + suf), o.array(c + suf)), c + suf
+
+
+def test_compute_forcing_pid(warmed):
+    """cuda_compute_forcing with the PID controller on gradP.z
+    (src/cuda_bluebottle.cu:3784-3806): gradP.z, the controller state and
+    f_z as the oracle computes them."""
+    _, s = warmed
+    if not s.case.particles:
+        pytest.skip("the controller averages particle accelerations")
+    from bluebottle_amd import runtime
+    o = _resync(s)
+    gains = (2.0, 0.5, 0.25)
+    pid_int = C.c_double(0.01)
+    pid_back = C.c_double(0.3)
+    o.s.Kp, o.s.Ki, o.s.Kd = gains
+    o.s.pid_int, o.s.pid_back = 0.01, 0.3
+    t = max(runtime.get_scalar("ttime", "d"), 1e-3)
+    runtime.set_scalar("ttime", t)
+    o.s.ttime = t
+    s.hip.cuda_compute_forcing(C.byref(pid_int), C.byref(pid_back), *gains)
+    o.call("compute_forcing")
+    s.pull()
+    gp = runtime.gvar("gradP", abi.gradP_struct)
+    assert gp.z == o.s.gradP.z
+    assert pid_int.value == o.s.pid_int and pid_back.value == o.s.pid_back
+    compare_fields(s, o, ("f_x", "f_y", "f_z"), 0.0)
 
 
 def test_compute_forcing(warmed):

@@ -479,3 +479,34 @@ def test_history_shift_copies():
     o.call("store_coeffs")
     for nm, nm00 in zip(COEFF_NAMES[:6], COEFF_NAMES[12:18]):
         assert np.array_equal(o.array(nm), o.array(nm00))
+
+
+def test_pid_forcing_one_step():
+    """The PID controller on gradP.z (src/cuda_bluebottle.cu:3784-3806),
+    one step computed by hand: two spheres of radius 0.1 and density 3 in a
+    unit box with rho_f = 1 make a suspension of volume fraction
+    phi = 2 (4/3) pi 0.1^3 and mixture density phi*3 + (1 - phi)*1; their
+    mean vertical acceleration a = (0.2 + 0.6)/2 = 0.4 feeds
+    gradP.z += (Kp a + Ki I/t + Kd (a - a_prev)) rho_mix with the running
+    integral I += a dt; the ramp of gradP.z is off while the controller
+    runs."""
+    case = configs.small_particle_case(n=8, nparts=2, radius_dx=0.8)
+    o, d = state(case)
+    for P, wdot in zip(o.s.parts[:2], (0.2, 0.6)):
+        P.r, P.rho, P.wdot = 0.1, 3.0, wdot
+    o.s.rho_f = 1.0
+    o.s.Kp, o.s.Ki, o.s.Kd = 2.0, 0.5, 0.25
+    o.s.pid_int, o.s.pid_back = 0.01, 0.3
+    o.s.dt, o.s.ttime, o.s.p_tDelay = 0.02, 0.8, 0.0
+    gp = o.s.gradP
+    gp.z, gp.zm, gp.za = -0.7, 5.0, 0.0
+    o.call("compute_forcing")
+    phi = 2 * (4.0 / 3.0) * math.pi * 0.1 ** 3
+    rho_mix = phi * 3.0 + (1 - phi) * 1.0
+    integ = 0.01 + 0.4 * 0.02                       # 0.018
+    ctrl = 2.0 * 0.4 + 0.5 * integ / 0.8 + 0.25 * (0.4 - 0.3)
+    want = -0.7 + ctrl * rho_mix
+    assert abs(o.s.gradP.z - want) < 1e-14
+    assert abs(o.s.pid_int - 0.018) < 1e-16
+    assert o.s.pid_back == 0.4
+    assert np.allclose(o.array("f_z"), -want, rtol=1e-14, atol=0)

@@ -435,3 +435,140 @@ def test_part_bc_p_masks_rhs():
     assert np.array_equal(got[keep], rhs[keep])
     assert np.all(got[~keep] == 0.0)
     assert (~keep).sum() > 50
+
+
+# ---------------------------------------------------------------------------
+# sphere-wall forces (src/bluebottle_kernel.cu:2875-3590): lubrication
+# against the asymptotic sphere-plane results, contact against Hertz
+# ---------------------------------------------------------------------------
+def _wall_sphere(h, a=None, wall="S", E=1.0e6, sigma=0.5, vel=(0.0, 0.0, 0.0),
+                 omega=(0.0, 0.0, 0.0), dt=1e-4):
+    """One sphere in the wall-bounded box (no-slip y walls, p Neumann), its
+    surface a distance h from the south (y = 0) or north (y = 1) wall, with
+    the given velocity and angular velocity (both time levels).  Returns the
+    particle record after move_parts_sub's collision pass (not integrated:
+    translating = rotating = 0)."""
+    case = configs.channel_case(n=(16, 16, 16), nparts=1)
+    P = case.particles[0]
+    if a is None:
+        a = P.r
+    P.r = a
+    P.x, P.z = 0.5, 0.5
+    P.y = a + h if wall == "S" else 1.0 - a - h
+    P.translating, P.rotating = 0, 0
+    P.E, P.sigma = E, sigma
+    # wall velocities zero, so that only the sphere's motion counts
+    zero = tuple((t, 0.0) for t, _ in case.bc_u)
+    case.bc_u = zero
+    o, d = _state(case)
+    o.s.dt = dt
+    Q = o.s.parts[0]
+    Q.u, Q.v, Q.w = vel
+    Q.u0, Q.v0, Q.w0 = vel
+    Q.ox, Q.oy, Q.oz = omega
+    Q.ox0, Q.oy0, Q.oz0 = omega
+    o.call("move_parts_sub")
+    Q = o.parts()[0]
+    rec = _Rec({f: getattr(Q, f) for f in ("y", "iFx", "iFy", "iFz", "iLx",
+                                          "iLy", "iLz")})
+    return o, rec, a
+
+
+class _Rec(dict):
+    """A copy of a particle record's fields (the oracle state's memory is
+    released with the state)."""
+    __getattr__ = dict.__getitem__
+
+
+def test_wall_squeeze_leading_order():
+    """Normal approach to a plane wall: F = -6 pi mu a^2 U / h at leading
+    order in h/a (Brenner 1961; Cox and Brenner 1967), repulsive on
+    approach for either wall."""
+    U = 0.04
+    for wall, sgn in (("S", -1.0), ("N", 1.0)):
+        h = 0.02 * 0.125
+        o, P, a = _wall_sphere(h, wall=wall, vel=(0.0, sgn * U, 0.0))
+        mu = o.s.mu
+        lead = -6.0 * math.pi * mu * a * a * (sgn * U) / h
+        # the next order is O(h/a) (the interaction length is a)
+        assert abs(P.iFy / lead - 1.0) < 2.0 * h / a
+        assert P.iFy * sgn < 0   # opposes the approach
+        assert P.iFx == 0.0 and P.iFz == 0.0
+        assert P.iLx == 0.0 and P.iLy == 0.0 and P.iLz == 0.0
+    # gaps below 1 % of the radius are treated as 1 % (the regularisation
+    # of the reference: the force stays finite at contact)
+    _, P1, a = _wall_sphere(0.01 * 0.125, vel=(0.0, -U, 0.0))
+    _, P2, _ = _wall_sphere(0.002 * 0.125, vel=(0.0, -U, 0.0))
+    assert abs(P1.iFy / P2.iFy - 1.0) < 1e-14
+
+
+def test_wall_tangential_log_coefficients_and_reciprocity():
+    """Tangential translation and rotation next to a plane wall: the force
+    and torque grow like ln(a/h) with the Goldman, Cox and Brenner (1967)
+    coefficients 6 pi mu a U (8/15) for the drag, 8 pi mu a^3 Omega (2/5)
+    for the rotational torque, and the coupling 8 pi mu a^2 (1/10) in both
+    directions (force from rotation equals torque from translation, the
+    symmetry of the resistance matrix).  A sphere translating along +x above
+    the south wall is turned as if rolling forward (torque about -z)."""
+    U, Om = 0.03, 0.7
+    h1, h2 = 0.02 * 0.125, 0.08 * 0.125
+    res = {}
+    for h in (h1, h2):
+        o, Pt, a = _wall_sphere(h, vel=(U, 0.0, 0.0))
+        _, Pr, _ = _wall_sphere(h, omega=(0.0, 0.0, Om))
+        res[h] = (Pt, Pr)
+    mu = o.s.mu
+    dlog = math.log(h2 / h1)
+    (Pt1, Pr1), (Pt2, Pr2) = res[h1], res[h2]
+    # drag from translation
+    assert abs((Pt2.iFx - Pt1.iFx) / dlog
+               / (6 * math.pi * mu * a * U * 8 / 15) - 1) < 1e-12
+    assert Pt1.iFx < 0
+    # rotational torque
+    assert abs((Pr2.iLz - Pr1.iLz) / dlog
+               / (8 * math.pi * mu * a ** 3 * Om * 2 / 5) - 1) < 1e-12
+    assert Pr1.iLz < 0
+    # coupling: torque from translation (rolling sense) and force from rotation
+    c_t = (Pt2.iLz - Pt1.iLz) / dlog / U
+    c_r = (Pr2.iFx - Pr1.iFx) / dlog / Om
+    assert abs(c_t / (8 * math.pi * mu * a * a / 10) - 1) < 1e-12
+    assert abs(c_r / c_t - 1) < 1e-12
+    assert Pt1.iLz < 0 and Pr1.iFx < 0
+    # nothing out of the plane of the motion
+    for P in (Pt1, Pr1):
+        assert P.iFz == 0.0 and P.iLx == 0.0 and P.iLy == 0.0
+
+
+def test_wall_hertz_contact_at_rest():
+    """A sphere at rest pressed into either wall by delta: the Hertz force
+    (4/3) E* sqrt(a) delta^(3/2) of a sphere on a flat of the same material,
+    1/E* = 2 (1 - sigma^2) / E, pushing it away from the wall."""
+    E, sigma = 3.0e5, 0.25
+    delta_rel = 0.02
+    for wall, away in (("S", 1.0), ("N", -1.0)):
+        o, P, a = _wall_sphere(-delta_rel * 0.125, wall=wall, E=E, sigma=sigma)
+        delta = delta_rel * 0.125
+        estar = E / (2.0 * (1.0 - sigma * sigma))
+        hertz = 4.0 / 3.0 * estar * math.sqrt(a) * delta ** 1.5
+        assert abs(P.iFy * away / hertz - 1.0) < 1e-12, (wall, P.iFy, hertz)
+        assert P.iFx == 0.0 and P.iFz == 0.0
+
+
+def test_spring_is_hookes_law():
+    """spring_parts (src/bluebottle_kernel.cu:3592-3617): a spring of rest
+    length l anchored at X0 pulls with -k (|d| - l) d/|d|, d = x - X0."""
+    case = _one_sphere()
+    P = case.particles[0]
+    P.x, P.y, P.z = 0.55, 0.47, 0.52
+    P.spring_k, P.spring_x, P.spring_y, P.spring_z = 40.0, 0.5, 0.5, 0.5
+    for l in (0.0, 0.03):
+        P.spring_l = l
+        o, d = _state(case)
+        o.s.dt = 1e-4
+        o.call("move_parts_sub")
+        Q = o.parts()[0]
+        got = np.array([Q.kFx, Q.kFy, Q.kFz])
+        dv = np.array([P.x - 0.5, P.y - 0.5, P.z - 0.5])
+        dn = np.linalg.norm(dv)
+        want = -P.spring_k * (dn - l) * dv / dn
+        assert np.allclose(got, want, rtol=1e-13, atol=0), (l, got, want)
