@@ -451,3 +451,94 @@ extern "C" real cuda_lamb_err(void)
   for(int i = 0; i < nparts; i++) if(g_bb.host_err[i] > mx) mx = g_bb.host_err[i];
   return mx;
 }
+
+/* ---------------------------------------------------------------------
+ * bb_lamb_err_noalloc: cuda_lamb_err for callers that cannot allocate
+ * device memory.  The per-particle errors and the result live in the
+ * file-scope array `scratch` (one copy per device, held until the process
+ * ends) instead of the library's workspace; the caller passes the six
+ * current and the six previous coefficient arrays.  No hipMalloc, hipFree
+ * or stream creation happens here.  Calls that overlap on one device race
+ * on `scratch`.
+ * ------------------------------------------------------------------- */
+#define BB_NOALLOC_MAX_PARTS 32768
+#define BB_NOALLOC_DEVICES 64
+
+static __device__ __align__(256) double scratch[BB_NOALLOC_MAX_PARTS + 64];
+
+/* device address of `scratch` on each device, set on the first call made
+ * with that device current */
+static double *g_scratch_dev[BB_NOALLOC_DEVICES];
+
+static double *scratch_ptr(int dev)
+{
+  if(!g_scratch_dev[dev]) {
+    void *ptr = NULL;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if(e != hipSuccess || !ptr) {
+      fprintf(stderr, "bluebottle_hip: bb_lamb_err_noalloc: no address for "
+        "its scratch on device %d (%s)\n", dev, hipGetErrorString(e));
+      exit(EXIT_FAILURE);
+    }
+    g_scratch_dev[dev] = (double *) ptr;
+  }
+  return g_scratch_dev[dev];
+}
+
+/* out[0] = max(FLT_MIN, err[0..n)), the host loop of cuda_lamb_err */
+__global__ void __launch_bounds__(64)
+k_err_max(int n, const double *__restrict__ err, double *__restrict__ out)
+{
+  __shared__ double sh[64];
+  double m = FLT_MIN;
+  for(int q = threadIdx.x; q < n; q += 64) if(err[q] > m) m = err[q];
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  if(threadIdx.x == 0) {
+    double r = FLT_MIN;
+    for(int q = 0; q < 64; q++) if(sh[q] > r) r = sh[q];
+    out[0] = r;
+  }
+}
+
+extern "C" real bb_lamb_err_noalloc(int np, int stride, real cut,
+  real *const cur[6], real *const prev[6])
+{
+  if(np <= 0) return 0.;
+  if(np > BB_NOALLOC_MAX_PARTS) {
+    fprintf(stderr, "bluebottle_hip: bb_lamb_err_noalloc: %d particles "
+      "exceed its scratch of %d; use cuda_lamb_err\n", np,
+      BB_NOALLOC_MAX_PARTS);
+    return -1.;
+  }
+  if(stride < 1 || stride > 21) {
+    fprintf(stderr, "bluebottle_hip: bb_lamb_err_noalloc: coefficient "
+      "stride %d outside 1..21 (Lamb order above 5)\n", stride);
+    return -1.;
+  }
+  int dev = 0;
+  BB_CHECK(hipGetDevice(&dev));
+  if(dev < 0 || dev >= BB_NOALLOC_DEVICES) {
+    fprintf(stderr, "bluebottle_hip: bb_lamb_err_noalloc: device %d beyond "
+      "the %d it keeps addresses for\n", dev, BB_NOALLOC_DEVICES);
+    return -1.;
+  }
+  double *sp = scratch_ptr(dev);
+  /* the library's stream when it has one on this device (ordered after
+   * the kernels that wrote the coefficients), else the null stream */
+  hipStream_t st = (g_bb.ready && g_bb.device == dev) ? g_bb.stream : 0;
+  k_lamb_err<<<np, 64, 0, st>>>(stride, cut, cur[0], prev[0], cur[1],
+    prev[1], cur[2], prev[2], cur[3], prev[3], cur[4], prev[4], cur[5],
+    prev[5], sp);
+  BB_CHECK_LAUNCH();
+  k_err_max<<<1, 64, 0, st>>>(np, sp, sp + BB_NOALLOC_MAX_PARTS);
+  BB_CHECK_LAUNCH();
+  size_t n = (size_t) stride * np * sizeof(real);
+  for(int a = 0; a < 6; a++)
+    BB_CHECK(hipMemcpyAsync(prev[a], cur[a], n, hipMemcpyDeviceToDevice, st));
+  double mx = 0.;
+  BB_CHECK(hipMemcpyAsync(&mx, sp + BB_NOALLOC_MAX_PARTS, sizeof(double),
+    hipMemcpyDeviceToHost, st));
+  BB_CHECK(hipStreamSynchronize(st));
+  return mx;
+}

@@ -422,6 +422,24 @@ void bb_device_free(void *ptr);
 void bb_copy_to_device(void *dev, const void *host, size_t bytes);
 void bb_copy_to_host(void *host, const void *dev, size_t bytes);
 
+/* No counterpart in the reference.  cuda_lamb_err for callers that cannot
+ * allocate device memory (code under stream capture, serving loops,
+ * harnesses that forbid hipMalloc): the largest relative change of the
+ * Lamb coefficients (the value cuda_lamb_err returns) for `np` particles
+ * of `stride` coefficients each, with the same copy of the current
+ * coefficients into the previous ones.  `cur` and `prev` hold the device
+ * arrays in the order p_nm re, im, phi_nm re, im, chi_nm re, im (the
+ * reference's _pnm_re, _pnm_re0, ... of src/particle.c:29-75 for the
+ * driver's state).  It allocates nothing: its per-particle scratch is a
+ * static device array of 32768 + 64 doubles (256.5 KiB, one per device,
+ * held until the process ends), so np is limited to 32768 and stride to
+ * 21; beyond either it prints why and returns -1.  Runs on the library
+ * stream when the library has one on the current device, else on the
+ * null stream, and returns when done.  Calls that overlap on one device
+ * race on the shared scratch. */
+real bb_lamb_err_noalloc(int np, int stride, real cut, real *const cur[6],
+  real *const prev[6]);
+
 /* ---------------------------------------------------------------------
  * Outside this library's path.  Exported so that the reference driver
  * links unchanged; each prints which path it belongs to and exits with

@@ -192,6 +192,7 @@ void run_grid(const char *name, dim3 g, dim3 b, const std::function<void()> &bod
 /* ---- runtime API ---- */
 hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
 hipError_t hipSetDevice(int) { return hipSuccess; }
+hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
 hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int)
 {
   memset(p, 0, sizeof(*p));
@@ -296,6 +297,19 @@ hipError_t hipFree(void *p)
   return hipSuccess;
 }
 static std::set<uintptr_t> g_host_allocs;
+
+hipError_t bb_emu_symbol_address(void **p, const void *sym, size_t bytes)
+{
+  static std::set<uintptr_t> seen;
+  {
+    std::lock_guard<std::mutex> l(g_reg_mu);
+    if(seen.count((uintptr_t) sym)) { *p = (void *) sym; return hipSuccess; }
+    seen.insert((uintptr_t) sym);
+  }
+  reg_add((void *) sym, bytes);
+  *p = (void *) sym;
+  return hipSuccess;
+}
 
 hipError_t hipHostMalloc(void **p, size_t n, unsigned)
 {

@@ -36,6 +36,8 @@
 #define __forceinline__ inline
 #define __shared__ static
 #define __launch_bounds__(...)
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) x
 
 struct dim3 {
   unsigned x, y, z;
@@ -74,6 +76,16 @@ struct hipDeviceProp_t {
 
 hipError_t hipGetDeviceCount(int *n);
 hipError_t hipSetDevice(int d);
+hipError_t hipGetDevice(int *d);
+/* the address of a file-scope __device__ array: registered as a device
+ * allocation of `bytes` on its first lookup (the emulated device's copy is
+ * the host array itself) */
+hipError_t bb_emu_symbol_address(void **p, const void *sym, size_t bytes);
+template <typename T>
+inline hipError_t hipGetSymbolAddress(void **p, const T &sym)
+{
+  return bb_emu_symbol_address(p, (const void *) &sym, sizeof(T));
+}
 hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int d);
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned flags);
 hipError_t hipStreamSynchronize(hipStream_t s);
