@@ -71,6 +71,7 @@ struct BBWorkspace {
   double *p2, *v2;          /* second buffers of p and v (ping-pong) */
   float *xf, *rf, *rsf, *pf, *vf, *tf, *bf;  /* FP32 inner-solver vectors */
   float *pf2, *vf2;
+  float *phf, *shf;         /* preconditioned p and s of the flexible solve */
   uint8_t *flags;
   double *partials;
   KrylovScalars *sc;        /* device */
@@ -109,6 +110,8 @@ void bb_ensure_context(void);
 /* Allocate / grow the Krylov workspace for the current Dom. */
 void bb_ensure_workspace(void);
 void bb_free_workspace(void);
+/* learned preconditioner: pair store and network (bb_poisson.hip) */
+void bb_learn_free(void);
 /* Crank-Nicolson solver workspace (bb_helmholtz.hip) */
 void bb_free_helmholtz(void);
 /* Particle workspace sized for nparts and the current grid. */

@@ -175,6 +175,8 @@ void bb_ensure_workspace(void)
   ws->vf2 = (float *) dmalloc(n * sizeof(float));
   ws->tf = (float *) dmalloc(n * sizeof(float));
   ws->bf = (float *) dmalloc(n * sizeof(float));
+  ws->phf = (float *) dmalloc(n * sizeof(float));
+  ws->shf = (float *) dmalloc(n * sizeof(float));
   ws->flags = (uint8_t *) dmalloc(n);
   ws->partials = (double *) dmalloc((size_t) ws->nblk * 4 * sizeof(double));
   ws->sc = (KrylovScalars *) dmalloc(sizeof(KrylovScalars));
@@ -190,7 +192,7 @@ void bb_free_workspace(void)
   BBWorkspace *ws = &g_bb.ws;
   void *ptrs[] = {ws->x, ws->r, ws->rs, ws->p, ws->v, ws->t, ws->b, ws->xf,
     ws->rf, ws->rsf, ws->pf, ws->vf, ws->tf, ws->bf, ws->flags, ws->partials, ws->sc,
-    ws->red, ws->p2, ws->v2, ws->pf2, ws->vf2};
+    ws->red, ws->p2, ws->v2, ws->pf2, ws->vf2, ws->phf, ws->shf};
   for(size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);
   if(ws->sc_host) BB_CHECK(hipHostFree(ws->sc_host));
   if(ws->red_host) BB_CHECK(hipHostFree(ws->red_host));
@@ -301,6 +303,7 @@ extern "C" void cuda_dom_free(void)
   }
   if(_rhs_p) { dfree(_rhs_p[0]); free(_rhs_p); _rhs_p = NULL; }
   if(_dom) { dfree(_dom[0]); free(_dom); _dom = NULL; }
+  bb_learn_free();
   bb_free_workspace();
   bb_free_helmholtz();
 }

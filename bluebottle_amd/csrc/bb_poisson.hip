@@ -35,6 +35,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include "bb_device.h"
+#include "bb_unet.h"
 
 #define PX 64
 #define PY 4
@@ -792,6 +793,193 @@ k_update_xr(PPGeom G, DinvTable tab, const KrylovScalars *sc,
 }
 
 /* ---------------------------------------------------------------------
+ * Flexible BiCGSTAB (right preconditioner M that may change between
+ * applications: the learned network).  Per iteration
+ *   p^ = M p;  v = A p^, <r*,v>;  alpha
+ *   s = r - alpha v, <s,s>         (s-convergence test)
+ *   s^ = M s;  t = A s^, <t,s>, <t,t>;  omega
+ *   x += alpha p^ + omega s^;  r = s - omega t, <r*,r>, <r,r>;  beta
+ *   p = r + beta (p - omega v)
+ * The products with A use the same z-marching sweep with the stored
+ * preconditioned vector as the operand.
+ * ------------------------------------------------------------------- */
+template<typename T>
+struct OperandPlain {
+  struct Raw { T p; uint8_t f; };
+  const T *__restrict__ p;
+  __device__ inline Raw load(const uint8_t *__restrict__ fl, long C) const
+  {
+    Raw w;
+    w.p = p[C];
+    w.f = fl[C];
+    return w;
+  }
+  __device__ inline CellVal val(const Raw &w, const double *) const
+  {
+    CellVal c;
+    c.f = w.f;
+    c.m = (double) w.p;
+    c.s = 0.;
+    return c;
+  }
+};
+
+template<typename T>
+struct OutTS {
+  T *__restrict__ t;
+  const T *__restrict__ s;
+  double ts, tt;
+  __device__ inline void operator()(long C, double y, double) {
+    const T tv = (T) y;
+    t[C] = tv;
+    const double sv = (double) s[C];
+    ts += (double) tv * sv;
+    tt += (double) tv * (double) tv;
+  }
+};
+
+/* v = A p^, partial <r*, v> */
+template<typename T>
+__global__ void __launch_bounds__(SX * SY)
+k_fspmv_v(PPGeom G, DinvTable tab, const KrylovScalars *sc,
+  const T *__restrict__ ph, const T *__restrict__ rs,
+  const uint8_t *__restrict__ fl, T *__restrict__ v,
+  double *__restrict__ part)
+{
+  if(sc->done) return;
+  __shared__ double sh[SX * SY / 64];
+  __shared__ double dinv[256];
+  __shared__ double pl[2][SY + 2][SX + 2];
+  load_dinv(tab, dinv);
+  const int lb = xcd_block(G.snblk);
+  const SweepTile t = sweep_tile(G, lb);
+  OperandPlain<T> op{ph};
+  OutP<T> out{rs, v, 0.};
+  sweep_z(G, t, fl, dinv, pl, op, out);
+  const double a = sweep_block_sum(out.acc, sh);
+  if(threadIdx.x == 0 && threadIdx.y == 0) part[lb] = a;
+}
+
+/* t = A s^, partials <t, s>, <t, t> */
+template<typename T>
+__global__ void __launch_bounds__(SX * SY)
+k_fspmv_t(PPGeom G, DinvTable tab, const KrylovScalars *sc,
+  const T *__restrict__ shat, const T *__restrict__ s,
+  const uint8_t *__restrict__ fl, T *__restrict__ t_out,
+  double *__restrict__ part)
+{
+  if(sc->done) return;
+  __shared__ double sh[SX * SY / 64];
+  __shared__ double dinv[256];
+  __shared__ double pl[2][SY + 2][SX + 2];
+  load_dinv(tab, dinv);
+  const int lb = xcd_block(G.snblk);
+  const SweepTile t = sweep_tile(G, lb);
+  OperandPlain<T> op{shat};
+  OutTS<T> out{t_out, s, 0., 0.};
+  sweep_z(G, t, fl, dinv, pl, op, out);
+  const double a = sweep_block_sum(out.ts, sh);
+  const double b = sweep_block_sum(out.tt, sh);
+  if(threadIdx.x == 0 && threadIdx.y == 0) { part[2*lb] = a; part[2*lb+1] = b; }
+}
+
+/* s = r - alpha v, partial <s, s> */
+template<typename T>
+__global__ void __launch_bounds__(256)
+k_form_s(PPGeom G, const KrylovScalars *sc, const T *__restrict__ r,
+  const T *__restrict__ v, T *__restrict__ s, double *__restrict__ part)
+{
+  if(sc->done) return;
+  __shared__ double sh[4];
+  const int lb = xcd_block(G.nblk);
+  TileIdx Ti = tile_of(G, lb);
+  const double alpha = sc->alpha;
+  double ss = 0.;
+  if(Ti.i < G.nx && Ti.j < G.ny) {
+    for(int k = Ti.k0; k < Ti.k1; k++) {
+      const int C = Ti.i + Ti.j*G.nx + k*G.nx*G.ny;
+      const T sv = (T) (1. * (double) r[C] + -alpha * (double) v[C]);
+      s[C] = sv;
+      ss += (double) sv * (double) sv;
+    }
+  }
+  const double b = bb_block_sum(ss, sh);
+  if(threadIdx.x + threadIdx.y * PX == 0) part[lb] = b;
+}
+
+__global__ void k_ff_s(int nblk, const double *part, KrylovScalars *sc)
+{
+  if(sc->done) return;
+  double o[1];
+  sum_partials<1>(part, nblk, o);
+  if(threadIdx.x == 0) {
+    double rs = sqrt(o[0]);
+    if(rs <= sc->tol) {
+      sc->s_conv = 1;
+      sc->resid = rs;
+      sc->converged = 1;
+    }
+  }
+}
+
+__global__ void k_ff_omega(int nblk, const double *part, KrylovScalars *sc)
+{
+  if(sc->done || sc->s_conv) return;
+  double o[2];
+  sum_partials<2>(part, nblk, o);
+  if(threadIdx.x == 0) sc->omega = o[0] / o[1];
+}
+
+/* x += alpha p^ + omega s^, r = s - omega t, partials <r*,r>, <r,r>;
+ * after an s-convergence only x += alpha p^ */
+template<typename T>
+__global__ void __launch_bounds__(256)
+k_fupdate_xr(PPGeom G, const KrylovScalars *sc, T *__restrict__ x,
+  const T *__restrict__ ph, const T *__restrict__ shat, T *__restrict__ r,
+  const T *__restrict__ s, const T *__restrict__ t, const T *__restrict__ rs,
+  double *__restrict__ part)
+{
+  if(sc->done) return;
+  __shared__ double sh[4];
+  const int lb = xcd_block(G.nblk);
+  TileIdx Ti = tile_of(G, lb);
+  const double alpha = sc->alpha, omega = sc->omega;
+  const int sconv = sc->s_conv;
+  double a0 = 0., a1 = 0.;
+  if(Ti.i < G.nx && Ti.j < G.ny) {
+    for(int k = Ti.k0; k < Ti.k1; k++) {
+      const int C = Ti.i + Ti.j*G.nx + k*G.nx*G.ny;
+      if(sconv) {
+        x[C] = (T) (1. * (double) x[C] + alpha * (double) ph[C]);
+        continue;
+      }
+      x[C] = (T) (1. * (double) x[C] + alpha * (double) ph[C]
+        + omega * (double) shat[C]);
+      const T rn = (T) (1. * (double) s[C] + -omega * (double) t[C]);
+      r[C] = rn;
+      a0 += (double) rs[C] * (double) rn;
+      a1 += (double) rn * (double) rn;
+    }
+  }
+  const double b0 = bb_block_sum(a0, sh);
+  const double b1 = bb_block_sum(a1, sh);
+  if(threadIdx.x + threadIdx.y * PX == 0) { part[2*lb] = b0; part[2*lb+1] = b1; }
+}
+
+/* p = r + beta (p - omega v) */
+template<typename T>
+__global__ void __launch_bounds__(256)
+k_fupdate_p(size_t n, const KrylovScalars *sc, T *__restrict__ p,
+  const T *__restrict__ r, const T *__restrict__ v)
+{
+  if(sc->done) return;
+  const double beta = sc->beta, bo = -sc->beta * sc->omega;
+  size_t stride = (size_t) gridDim.x * blockDim.x;
+  for(size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+    p[q] = (T) (1. * (double) r[q] + beta * (double) p[q] + bo * (double) v[q]);
+}
+
+/* ---------------------------------------------------------------------
  * Mean removal and copy back: x *= nrm; x -= mean(x); phi(interior) = x
  * ------------------------------------------------------------------- */
 __global__ void __launch_bounds__(256)
@@ -990,6 +1178,232 @@ static void solve_fp64(const PPGeom &G, const DinvTable &tab)
   g_bb.stats.converged = h->converged;
 }
 
+/* ---------------------------------------------------------------------
+ * Learned preconditioner: the flexible inner solve, the store of
+ * (residual, correction) pairs produced by the Jacobi inner solves, and
+ * the training loop (network in bb_unet.hip).
+ * ------------------------------------------------------------------- */
+struct LearnState {
+  int active;              /* network trained and in use */
+  int stall_window;        /* fallback test every this many iterations */
+  double stall_drop;       /* ... requires |r| to fall below this ratio */
+  /* pair store */
+  int cap, count, next;
+  float *pr, *pe;          /* cap x n each */
+  uint8_t *pf;             /* cap x n flags */
+  size_t n;
+  /* statistics */
+  long applications, fallbacks, flex_iters, jacobi_iters;
+  double train_seconds;
+};
+
+static LearnState g_learn = {0, 5, 0.9, 0, 0, 0, NULL, NULL, NULL, 0,
+  0, 0, 0, 0, 0.};
+
+static void learn_free_pairs(void)
+{
+  if(g_learn.pr) BB_CHECK(hipFree(g_learn.pr));
+  if(g_learn.pe) BB_CHECK(hipFree(g_learn.pe));
+  if(g_learn.pf) BB_CHECK(hipFree(g_learn.pf));
+  g_learn.pr = g_learn.pe = NULL;
+  g_learn.pf = NULL;
+  g_learn.cap = g_learn.count = g_learn.next = 0;
+}
+
+/* keep (b, x) of the Jacobi inner solve just finished: the normalised
+ * residual and its correction, with the operator flags of this solve */
+static void collect_pair(BBWorkspace *ws)
+{
+  if(g_learn.cap <= 0 || g_learn.n != ws->n) return;
+  hipStream_t st = g_bb.stream;
+  size_t n = ws->n;
+  int slot = g_learn.next;
+  BB_CHECK(hipMemcpyAsync(g_learn.pr + slot * n, ws->bf, n * sizeof(float),
+    hipMemcpyDeviceToDevice, st));
+  BB_CHECK(hipMemcpyAsync(g_learn.pe + slot * n, ws->xf, n * sizeof(float),
+    hipMemcpyDeviceToDevice, st));
+  BB_CHECK(hipMemcpyAsync(g_learn.pf + slot * n, ws->flags, n,
+    hipMemcpyDeviceToDevice, st));
+  g_learn.next = (slot + 1) % g_learn.cap;
+  if(g_learn.count < g_learn.cap) g_learn.count++;
+}
+
+static void apply_net(const float *in, float *out)
+{
+  bb_unet_apply_dev(in, g_bb.ws.flags, out, g_bb.stream);
+  g_learn.applications++;
+}
+
+/* Flexible BiCGSTAB on the FP32 inner system A e = b (e from 0), judged
+ * by the same relative tolerance; falls back to Jacobi BiCGSTAB from the
+ * current iterate when |r| drops by less than 10 % over 5 iterations. */
+static void flexible_run(const PPGeom &G, const DinvTable &tab, double rel_tol,
+  int max_iter)
+{
+  BBWorkspace *ws = &g_bb.ws;
+  hipStream_t st = g_bb.stream;
+  dim3 blk(PX, PY), sblk(SX, SY);
+  size_t n = ws->n;
+  int eb = elementwise_blocks(n);
+  float *b = ws->bf, *x = ws->xf, *r = ws->rf, *rs = ws->rsf, *p = ws->pf;
+  float *v = ws->vf, *sv = ws->pf2, *t = ws->tf, *ph = ws->phf, *sh = ws->shf;
+  if(G.snblk > ws->nblk || G.nblk > ws->nblk) {
+    fprintf(stderr, "bluebottle_hip: %d sweep tiles exceed the %d reduction "
+      "slots\n", G.snblk, ws->nblk);
+    exit(EXIT_FAILURE);
+  }
+  k_init<float><<<G.nblk, blk, 0, st>>>(G, tab, ws->sc, b, x, ws->flags, r,
+    rs, p, v, ws->partials, 0);
+  BB_CHECK_LAUNCH();
+  k_f_init<<<1, FIN_THREADS, 0, st>>>(G.nblk, ws->partials, ws->sc, rel_tol,
+    max_iter);
+  BB_CHECK_LAUNCH();
+  BB_CHECK(hipMemcpyAsync(ws->sc_host, ws->sc, sizeof(KrylovScalars),
+    hipMemcpyDeviceToHost, st));
+  BB_CHECK(hipStreamSynchronize(st));
+  if(ws->sc_host->done) return;
+  double hist[64];
+  int it;
+  for(it = 0; it < max_iter; it++) {
+    apply_net(p, ph);
+    k_fspmv_v<float><<<G.snblk, sblk, 0, st>>>(G, tab, ws->sc, ph, rs,
+      ws->flags, v, ws->partials);
+    BB_CHECK_LAUNCH();
+    k_f1<<<1, FIN_THREADS, 0, st>>>(G.snblk, ws->partials, ws->sc);
+    BB_CHECK_LAUNCH();
+    k_form_s<float><<<G.nblk, blk, 0, st>>>(G, ws->sc, r, v, sv, ws->partials);
+    BB_CHECK_LAUNCH();
+    k_ff_s<<<1, FIN_THREADS, 0, st>>>(G.nblk, ws->partials, ws->sc);
+    BB_CHECK_LAUNCH();
+    apply_net(sv, sh);
+    k_fspmv_t<float><<<G.snblk, sblk, 0, st>>>(G, tab, ws->sc, sh, sv,
+      ws->flags, t, ws->partials);
+    BB_CHECK_LAUNCH();
+    k_ff_omega<<<1, FIN_THREADS, 0, st>>>(G.snblk, ws->partials, ws->sc);
+    BB_CHECK_LAUNCH();
+    k_fupdate_xr<float><<<G.nblk, blk, 0, st>>>(G, ws->sc, x, ph, sh, r, sv,
+      t, rs, ws->partials);
+    BB_CHECK_LAUNCH();
+    k_f3<<<1, FIN_THREADS, 0, st>>>(G.nblk, ws->partials, ws->sc);
+    BB_CHECK_LAUNCH();
+    k_fupdate_p<float><<<eb, 256, 0, st>>>(n, ws->sc, p, r, v);
+    BB_CHECK_LAUNCH();
+    BB_CHECK(hipMemcpyAsync(ws->sc_host, ws->sc, sizeof(KrylovScalars),
+      hipMemcpyDeviceToHost, st));
+    BB_CHECK(hipStreamSynchronize(st));
+    hist[it % 64] = ws->sc_host->resid;
+    if(ws->sc_host->done) break;
+    const int w = g_learn.stall_window;
+    if(it + 1 >= w && (it + 1) % w == 0) {
+      double before = it + 1 == w ? ws->sc_host->tol / rel_tol
+                                  : hist[(it - w) % 64];
+      if(ws->sc_host->resid > g_learn.stall_drop * before) {
+        /* stalled: Jacobi BiCGSTAB from the current iterate */
+        int done_it = ws->sc_host->iter;
+        g_learn.flex_iters += done_it;
+        g_learn.fallbacks++;
+        int kinds[4] = {BB_K_SPMV_P32, BB_K_SPMV_S32, BB_K_UPDATE_XR32,
+          BB_K_UPDATE_P32};
+        bicgstab_run<float>(G, tab, b, x, r, rs, p, v, ws->pf2, ws->vf2, t,
+          rel_tol, max_iter - done_it, 0, g_poll, kinds);
+        g_learn.jacobi_iters += ws->sc_host->iter;
+        ws->sc_host->iter += done_it;
+        return;
+      }
+    }
+  }
+  g_learn.flex_iters += ws->sc_host->iter;
+}
+
+/* ---- C ABI of the learned form (no reference counterpart) ---- */
+extern "C" void bb_learn_collect(int capacity)
+{
+  bb_ensure_workspace();
+  learn_free_pairs();
+  if(capacity <= 0) return;
+  size_t n = g_bb.ws.n;
+  g_learn.n = n;
+  BB_CHECK(hipMalloc(&g_learn.pr, (size_t) capacity * n * sizeof(float)));
+  BB_CHECK(hipMalloc(&g_learn.pe, (size_t) capacity * n * sizeof(float)));
+  BB_CHECK(hipMalloc(&g_learn.pf, (size_t) capacity * n));
+  g_learn.cap = capacity;
+}
+
+extern "C" int bb_learn_pairs(void) { return g_learn.count; }
+
+/* Train the network on the stored pairs: `iters` Adam steps at learning
+ * rate `lr`, pairs drawn from a splitmix64 stream seeded with `seed`.
+ * Returns the mean loss of the last tenth of the steps (-1 without pairs
+ * or a suitable grid).  Afterwards the learned form uses the network. */
+extern "C" real bb_learn_train(int iters, real lr, unsigned long seed)
+{
+  bb_ensure_workspace();
+  if(g_learn.count <= 0 || !bb_unet_ready()) return -1.;
+  hipStream_t st = g_bb.stream;
+  size_t n = g_learn.n;
+  uint64_t sd = seed;
+  double tail = 0.;
+  int ntail = 0;
+  hipEvent_t e0, e1;
+  BB_CHECK(hipEventCreate(&e0));
+  BB_CHECK(hipEventCreate(&e1));
+  BB_CHECK(hipEventRecord(e0, st));
+  for(int i = 0; i < iters; i++) {
+    uint64_t z = (sd += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    int k = (int) (z % (uint64_t) g_learn.count);
+    double loss = bb_unet_train_pair(g_learn.pr + k * n, g_learn.pe + k * n,
+      g_learn.pf + k * n, (float) lr, 1, st);
+    if(i >= iters - (iters + 9) / 10) { tail += loss; ntail++; }
+  }
+  BB_CHECK(hipEventRecord(e1, st));
+  BB_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  BB_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  BB_CHECK(hipEventDestroy(e0));
+  BB_CHECK(hipEventDestroy(e1));
+  g_learn.train_seconds += 1e-3 * ms;
+  g_learn.active = 1;
+  return ntail ? tail / ntail : -1.;
+}
+
+/* use (1) or ignore (0) the network in the learned form */
+extern "C" void bb_learn_enable(int on)
+{
+  g_learn.active = on && bb_unet_ready();
+}
+
+extern "C" void bb_learn_set_fallback(int window, real drop)
+{
+  if(window > 0 && window <= 60) g_learn.stall_window = window;
+  if(drop > 0) g_learn.stall_drop = drop;
+}
+
+extern "C" void bb_learn_stats(long *applications, long *fallbacks,
+  long *flex_iters, long *jacobi_iters, real *train_seconds)
+{
+  if(applications) *applications = g_learn.applications;
+  if(fallbacks) *fallbacks = g_learn.fallbacks;
+  if(flex_iters) *flex_iters = g_learn.flex_iters;
+  if(jacobi_iters) *jacobi_iters = g_learn.jacobi_iters;
+  if(train_seconds) *train_seconds = g_learn.train_seconds;
+}
+
+extern "C" void bb_learn_reset_stats(void)
+{
+  g_learn.applications = g_learn.fallbacks = 0;
+  g_learn.flex_iters = g_learn.jacobi_iters = 0;
+}
+
+void bb_learn_free(void)
+{
+  learn_free_pairs();
+  g_learn.active = 0;
+  bb_unet_free();
+}
+
 /* FP64 refinement around FP32 BiCGSTAB.  Inner solves run to a relative
  * reduction of `inner_tol` (or `inner_max` iterations); each outer step
  * recomputes the true residual in FP64. */
@@ -1056,8 +1470,13 @@ static void solve_mixed(const PPGeom &G, const DinvTable &tab)
     if(need > inner_tol) inner_tol = need;
     int imax = g_inner_max;
     if(imax > pp_max_iter - total_inner) imax = pp_max_iter - total_inner;
-    bicgstab_run<float>(G, tab, ws->bf, ws->xf, ws->rf, ws->rsf, ws->pf, ws->vf,
-      ws->pf2, ws->vf2, ws->tf, inner_tol, imax, 0, g_poll, kinds);
+    if(g_bb.solver_mode == BB_SOLVER_LEARNED && g_learn.active) {
+      flexible_run(G, tab, inner_tol, imax);
+    } else {
+      bicgstab_run<float>(G, tab, ws->bf, ws->xf, ws->rf, ws->rsf, ws->pf,
+        ws->vf, ws->pf2, ws->vf2, ws->tf, inner_tol, imax, 0, g_poll, kinds);
+      collect_pair(ws);
+    }
     total_inner += ws->sc_host->iter + (ws->sc_host->s_conv ? 1 : 0);
     bb_prof_begin(BB_K_CORRECT64);
     k_correct64<<<eb, 256, 0, st>>>(n, rn, ws->xf, ws->x);
@@ -1093,7 +1512,8 @@ extern "C" void cuda_PP_bicgstab(int rank)
   k_f_norm<<<1, 256, 0, st>>>(G.nblk, ws->partials, ws->sc);
   BB_CHECK_LAUNCH();
 
-  if(g_bb.solver_mode == BB_SOLVER_MIXED) {
+  if(g_bb.solver_mode == BB_SOLVER_MIXED ||
+     g_bb.solver_mode == BB_SOLVER_LEARNED) {
     /* a zero FP64 vector for the |b| pass of the refinement loop */
     BB_CHECK(hipMemsetAsync(ws->t, 0, ws->n * sizeof(double), st));
     solve_mixed(G, tab);

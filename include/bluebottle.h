@@ -339,6 +339,10 @@ real cuda_lamb_err(void);
 /* Pressure-Poisson solver mode. */
 #define BB_SOLVER_FP64  0  /* plain FP64 Jacobi-BiCGSTAB (reference form) */
 #define BB_SOLVER_MIXED 1  /* FP32 Jacobi-BiCGSTAB inside FP64 refinement */
+/* FP32 flexible BiCGSTAB preconditioned by the learned U-Net inside FP64
+ * refinement (Jacobi BiCGSTAB until a network is trained, and from any
+ * iterate where |r| falls by less than 10 % over 5 iterations) */
+#define BB_SOLVER_LEARNED 2
 void bb_set_solver_mode(int mode);
 int bb_get_solver_mode(void);
 
@@ -421,6 +425,41 @@ void *bb_device_alloc(size_t bytes);
 void bb_device_free(void *ptr);
 void bb_copy_to_device(void *dev, const void *host, size_t bytes);
 void bb_copy_to_host(void *host, const void *dev, size_t bytes);
+
+/* ---------------------------------------------------------------------
+ * The learned preconditioner (no counterpart in the reference, whose
+ * Poisson solve is diagonally preconditioned: src/cuda_bicgstab.cu:
+ * 1016-1018).  A 3-level 3-D U-Net (8/16/32 channels, 3x3x3 convolutions,
+ * BF16 weights and activations, FP32 accumulation), trained with Adam on
+ * (normalised residual, correction) pairs the mixed form's Jacobi inner
+ * solves produce.
+ * ------------------------------------------------------------------- */
+/* keep the pairs of the next `capacity` inner solves (0: stop and free) */
+void bb_learn_collect(int capacity);
+int bb_learn_pairs(void);
+/* `iters` Adam steps at rate `lr` on pairs drawn with `seed`; returns the
+ * mean loss |A (M r - e)|^2 / |r|^2 of the last tenth (-1: nothing to
+ * train on, or grid sizes not divisible by 4).  Enables the network. */
+real bb_learn_train(int iters, real lr, unsigned long seed);
+void bb_learn_enable(int on);
+/* fallback test: every `window` iterations, |r| must fall below `drop`
+ * times its value `window` iterations earlier (defaults 5 and 0.9) */
+void bb_learn_set_fallback(int window, real drop);
+void bb_learn_stats(long *applications, long *fallbacks, long *flex_iters,
+  long *jacobi_iters, real *train_seconds);
+void bb_learn_reset_stats(void);
+/* network weights (FP32 masters, layer by layer, each [cout][cin][27]) */
+long bb_unet_num_params(void);
+void bb_unet_init(unsigned long seed);
+void bb_unet_set_params(const float *host, long n);
+void bb_unet_get_params(float *host, long n);
+void bb_unet_get_grads(float *host, long n);
+/* out = M(r) for device vectors of Gcc.s3 floats and one operator-flag
+ * byte per cell (BB_F* bits, solid 64) */
+void bb_unet_apply(const float *r, const unsigned char *flags, float *out);
+/* loss of one pair and the weight gradients; lr > 0 also steps Adam */
+real bb_unet_loss_grad(const float *r, const float *e,
+  const unsigned char *flags, real lr);
 
 /* No counterpart in the reference.  cuda_lamb_err for callers that cannot
  * allocate device memory (code under stream capture, serving loops,
