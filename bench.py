@@ -131,6 +131,94 @@ def run_mode(s, mode, steps, warmup, prof_steps, runtime):
     }
 
 
+def prepare_learned(s, warm_steps, iters, lr, capacity):
+    """In-situ training of the learned preconditioner on this state: mixed
+    steps keep the (residual, correction) pairs of their Jacobi inner
+    solves, then Adam trains the network on them."""
+    import ctypes as C
+    hip = s.hip
+    hip.bb_learn_collect(capacity)
+    s.set_solver(1)
+    t0 = time.perf_counter()
+    for i in range(warm_steps):
+        s.step()
+        log("learned: collection step %d done, %d pairs" % (
+            i, hip.bb_learn_pairs()))
+    hip.bb_synchronize()
+    t_collect = time.perf_counter() - t0
+    npairs = hip.bb_learn_pairs()
+    hip.bb_unet_init(20251202)
+    base = [hip.bb_learn_pair_loss(k) for k in range(npairs)]
+    t0 = time.perf_counter()
+    loss = hip.bb_learn_train(iters, lr, 7)
+    hip.bb_synchronize()
+    t_train = time.perf_counter() - t0
+    after = [hip.bb_learn_pair_loss(k) for k in range(npairs)]
+    log("learned: %d pairs, %d Adam steps in %.2f s, pair loss %.3e -> %.3e" % (
+        npairs, iters, t_train, sum(base) / max(len(base), 1),
+        sum(after) / max(len(after), 1)))
+    hip.bb_learn_collect(0)
+    return {"pairs": npairs, "collection_steps": warm_steps,
+            "collection_seconds": round(t_collect, 3),
+            "adam_steps": iters, "lr": lr,
+            "train_seconds": round(t_train, 3),
+            "pair_loss_jacobi": round(sum(base) / max(len(base), 1), 6),
+            "pair_loss_trained": round(sum(after) / max(len(after), 1), 6),
+            "last_tenth_loss": round(loss, 6)}
+
+
+def time_network(s, reps, runtime):
+    """HIP-event times of the network's application on this grid (random
+    input, all-fluid flags): the whole application and its full-resolution
+    8->8 convolutions (two per application)."""
+    import ctypes as C
+    import numpy as np
+    hip = s.hip
+    d = s.dom
+    n = d.xn * d.yn * d.zn
+    r = np.random.default_rng(1).standard_normal(n).astype(np.float32)
+    fl = np.full(n, 63, np.uint8)
+    dr = hip.bb_device_alloc(r.nbytes)
+    df = hip.bb_device_alloc(fl.nbytes)
+    do = hip.bb_device_alloc(r.nbytes)
+    hip.bb_copy_to_device(dr, r.ctypes.data, r.nbytes)
+    hip.bb_copy_to_device(df, fl.ctypes.data, fl.nbytes)
+    hip.bb_unet_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    hip.bb_unet_apply(dr, df, do)
+    hip.bb_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hip.bb_unet_apply(dr, df, do)      # returns when done
+    app_s = (time.perf_counter() - t0) / reps
+    hip.bb_prof_reset()
+    hip.bb_prof_enable(1)
+    for _ in range(reps):
+        hip.bb_unet_apply(dr, df, do)
+    hip.bb_synchronize()
+    hip.bb_prof_enable(0)
+    ms = C.c_double()
+    cnt = C.c_long()
+    hip.bb_prof_read(runtime.PROF_KINDS.index("unet_conv0"), C.byref(ms),
+                     C.byref(cnt))
+    for p in (dr, df, do):
+        hip.bb_device_free(p)
+    conv_ms, conv_cnt = ms.value, cnt.value
+    avg = conv_ms / max(conv_cnt, 1) * 1e-3
+    flops = 2.0 * 27 * 8 * 8 * n
+    nbytes = (16 + 16) * n
+    return {
+        "application_ms": round(app_s * 1e3, 4),
+        "conv0_kernel": "k_conv3<8,8> (full resolution, BF16 MFMA)",
+        "conv0_avg_us": round(avg * 1e6, 2),
+        "bound": "mfma", "achieved": round(flops / avg / 1e12, 2),
+        "peak": 2500.0, "unit": "TFLOP/s",
+        "frac": round(flops / avg / 1e12 / 2500.0, 4),
+        "flops_per_launch": flops,
+        "hbm_GBs": round(nbytes / avg / 1e9, 1),
+        "hbm_frac": round(nbytes / avg / 1e9 / HBM_PEAK_GBS, 4),
+    }
+
+
 def roofline(prof, nbytes):
     best = None
     for name, (ms, cnt) in prof.items():
@@ -280,7 +368,12 @@ def main():
                     help="sediment: 256^3 with --nparts spheres (the metric's "
                          "configuration); tg: the same box as a sphere-free "
                          "Taylor-Green flow")
-    ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--modes", default="fp64,mixed,learned")
+    ap.add_argument("--learn-collect-steps", type=int, default=3,
+                    help="mixed steps whose inner solves supply the training "
+                         "pairs of the learned form")
+    ap.add_argument("--learn-iters", type=int, default=400)
+    ap.add_argument("--learn-lr", type=float, default=1e-3)
     ap.add_argument("--prof-steps", type=int, default=1,
                     help="steps after the timed ones with per-launch event "
                          "timing (the kernel table and the roofline)")
@@ -325,7 +418,9 @@ def main():
     ncell = xn * yn * zn
     nbytes = kernel_bytes(ncell, xn, yn, zn)
     modes = {}
-    names = {"fp64": 0, "mixed": 1}
+    names = {"fp64": 0, "mixed": 1, "learned": 2}
+    learn = None
+    net_roof = None
     cpu = None
     copy_gbs = None
     for m in args.modes.split(","):
@@ -346,8 +441,28 @@ def main():
             # the achievable streaming rate on this device: a plain 1 GiB copy
             copy_gbs = s.hip.bb_copy_bandwidth(1 << 30, 10)
             log("plain HBM copy: %.0f GB/s" % copy_gbs)
+        if m == "learned":
+            learn = prepare_learned(s, args.learn_collect_steps,
+                                    args.learn_iters, args.learn_lr, 32)
+            s.hip.bb_learn_reset_stats()
         modes[m] = run_mode(s, names[m], args.steps, args.warmup,
                             args.prof_steps, runtime)
+        if m == "learned":
+            import ctypes as C
+            st = [C.c_long() for _ in range(5)]
+            ts = C.c_double()
+            s.hip.bb_learn_stats(*[C.byref(x) for x in st], C.byref(ts))
+            learn.update({"network_applications": st[0].value,
+                          "fallbacks": st[1].value,
+                          "flexible_iters": st[2].value,
+                          "jacobi_iters_after_fallback": st[3].value,
+                          "inner_solves_skipping_network": st[4].value})
+            modes[m]["learned"] = learn
+            net_roof = time_network(s, 10, runtime)
+            log("network: %.3f ms per application, conv0 %.1f us "
+                "(%.1f TFLOP/s, %.0f GB/s)" % (
+                    net_roof["application_ms"], net_roof["conv0_avg_us"],
+                    net_roof["achieved"], net_roof["hbm_GBs"]))
         log("%s: %.4f timesteps/s, %.2f Lamb it/step, %.1f Krylov it/solve" % (
             m, modes[m]["timesteps_per_s"], modes[m]["lamb_iters_per_step"],
             modes[m]["krylov_iters_per_solve"]))
@@ -355,7 +470,8 @@ def main():
             log("timing the CPU oracle sample")
             cpu = cpu_baseline(s, modes["fp64"], args.krylov_sample, desc)
             log("cpu sample done")
-    head = "mixed" if "mixed" in modes else list(modes)[0]
+    head = ("learned" if "learned" in modes else
+            "mixed" if "mixed" in modes else list(modes)[0])
     H = modes[head]
     roof, table = roofline(H["prof"], nbytes)
     if roof is not None and args.n == 256 and args.nparts == 1000:
@@ -373,25 +489,33 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64+f32" if head == "mixed" else "f64",
+        "dtype": {"learned": "f64+f32+bf16", "mixed": "f64+f32"}.get(head,
+                                                                   "f64"),
         "data": "synthetic",
         "config": {
             "workload": desc,
-            "solver": head + " BiCGSTAB, Jacobi preconditioner",
+            "solver": {
+                "learned": "FP64 refinement around FP32 flexible BiCGSTAB "
+                           "preconditioned by the in-situ trained U-Net, "
+                           "Jacobi fallback",
+                "mixed": "FP64 refinement around FP32 Jacobi BiCGSTAB",
+                "fp64": "FP64 Jacobi BiCGSTAB"}[head],
             "lamb_max_iter": case.lamb_max_iter,
         },
         "modes": {m: {k: (round(v, 5) if isinstance(v, float) else v)
                       for k, v in d.items() if k != "prof"}
                   for m, d in modes.items()},
         "roofline": roof,
+        "roofline_mfma": net_roof,
         "hbm_copy_GBs": round(copy_gbs, 1),
         "kernels": table,
         "cpu_baseline": cpu,
     }
-    if "fp64" in modes and "mixed" in modes:
-        out["mixed_over_fp64"] = round(
-            modes["mixed"]["timesteps_per_s"] / modes["fp64"]["timesteps_per_s"],
-            3)
+    for m in ("mixed", "learned"):
+        if "fp64" in modes and m in modes:
+            out[m + "_over_fp64"] = round(
+                modes[m]["timesteps_per_s"] / modes["fp64"]["timesteps_per_s"],
+                3)
     s.close()
     print(json.dumps(out), flush=True)
 

@@ -131,7 +131,8 @@ enum {
   BB_K_UPDATE_XR, BB_K_UPDATE_P, BB_K_FINAL, BB_K_RESID64, BB_K_CORRECT64,
   BB_K_SPMV_P32, BB_K_SPMV_S32, BB_K_UPDATE_XR32, BB_K_UPDATE_P32,
   BB_K_PROJECT, BB_K_UPDATEP, BB_K_FINDDT, BB_K_CAGE, BB_K_PARTBC,
-  BB_K_QUAD, BB_K_MOVE, BB_K_MISC, BB_K_NKINDS
+  BB_K_QUAD, BB_K_MOVE, BB_K_MISC, BB_K_UNET, BB_K_UNET_CONV0, BB_K_FSPMV,
+  BB_K_FVEC, BB_K_NKINDS
 };
 static_assert((int) BB_K_NKINDS == (int) BB_PROF_NKINDS &&
   (int) BB_K_SPMV_S32 == (int) BB_PROF_SPMV_S32 &&

@@ -1192,13 +1192,19 @@ struct LearnState {
   float *pr, *pe;          /* cap x n each */
   uint8_t *pf;             /* cap x n flags */
   size_t n;
+  /* after a fallback the next `skip` inner solves use Jacobi directly,
+   * then the network is tried again; `skip` starts at skip_after_stall and
+   * doubles with every fallback (up to 1024), and returns to
+   * skip_after_stall after a solve of at least one stall window that the
+   * network finishes, or a retraining */
+  int skip_after_stall, skip_left, skip_next;
   /* statistics */
-  long applications, fallbacks, flex_iters, jacobi_iters;
+  long applications, fallbacks, flex_iters, jacobi_iters, skipped;
   double train_seconds;
 };
 
 static LearnState g_learn = {0, 5, 0.9, 0, 0, 0, NULL, NULL, NULL, 0,
-  0, 0, 0, 0, 0.};
+  8, 0, 8, 0, 0, 0, 0, 0, 0.};
 
 static void learn_free_pairs(void)
 {
@@ -1215,6 +1221,9 @@ static void learn_free_pairs(void)
 static void collect_pair(BBWorkspace *ws)
 {
   if(g_learn.cap <= 0 || g_learn.n != ws->n) return;
+  /* only corrections solved to 1e-4 relative (|b| = 1): the outer loop
+   * lets an inner solve stop early when the outer test needs less */
+  if(!(ws->sc_host->resid <= 1e-4)) return;
   hipStream_t st = g_bb.stream;
   size_t n = ws->n;
   int slot = g_learn.next;
@@ -1230,7 +1239,9 @@ static void collect_pair(BBWorkspace *ws)
 
 static void apply_net(const float *in, float *out)
 {
+  bb_prof_begin(BB_K_UNET);
   bb_unet_apply_dev(in, g_bb.ws.flags, out, g_bb.stream);
+  bb_prof_end(BB_K_UNET);
   g_learn.applications++;
 }
 
@@ -1266,28 +1277,38 @@ static void flexible_run(const PPGeom &G, const DinvTable &tab, double rel_tol,
   int it;
   for(it = 0; it < max_iter; it++) {
     apply_net(p, ph);
+    bb_prof_begin(BB_K_FSPMV);
     k_fspmv_v<float><<<G.snblk, sblk, 0, st>>>(G, tab, ws->sc, ph, rs,
       ws->flags, v, ws->partials);
     BB_CHECK_LAUNCH();
+    bb_prof_end(BB_K_FSPMV);
     k_f1<<<1, FIN_THREADS, 0, st>>>(G.snblk, ws->partials, ws->sc);
     BB_CHECK_LAUNCH();
+    bb_prof_begin(BB_K_FVEC);
     k_form_s<float><<<G.nblk, blk, 0, st>>>(G, ws->sc, r, v, sv, ws->partials);
     BB_CHECK_LAUNCH();
+    bb_prof_end(BB_K_FVEC);
     k_ff_s<<<1, FIN_THREADS, 0, st>>>(G.nblk, ws->partials, ws->sc);
     BB_CHECK_LAUNCH();
     apply_net(sv, sh);
+    bb_prof_begin(BB_K_FSPMV);
     k_fspmv_t<float><<<G.snblk, sblk, 0, st>>>(G, tab, ws->sc, sh, sv,
       ws->flags, t, ws->partials);
     BB_CHECK_LAUNCH();
+    bb_prof_end(BB_K_FSPMV);
     k_ff_omega<<<1, FIN_THREADS, 0, st>>>(G.snblk, ws->partials, ws->sc);
     BB_CHECK_LAUNCH();
+    bb_prof_begin(BB_K_FVEC);
     k_fupdate_xr<float><<<G.nblk, blk, 0, st>>>(G, ws->sc, x, ph, sh, r, sv,
       t, rs, ws->partials);
     BB_CHECK_LAUNCH();
+    bb_prof_end(BB_K_FVEC);
     k_f3<<<1, FIN_THREADS, 0, st>>>(G.nblk, ws->partials, ws->sc);
     BB_CHECK_LAUNCH();
+    bb_prof_begin(BB_K_FVEC);
     k_fupdate_p<float><<<eb, 256, 0, st>>>(n, ws->sc, p, r, v);
     BB_CHECK_LAUNCH();
+    bb_prof_end(BB_K_FVEC);
     BB_CHECK(hipMemcpyAsync(ws->sc_host, ws->sc, sizeof(KrylovScalars),
       hipMemcpyDeviceToHost, st));
     BB_CHECK(hipStreamSynchronize(st));
@@ -1308,11 +1329,17 @@ static void flexible_run(const PPGeom &G, const DinvTable &tab, double rel_tol,
           rel_tol, max_iter - done_it, 0, g_poll, kinds);
         g_learn.jacobi_iters += ws->sc_host->iter;
         ws->sc_host->iter += done_it;
+        g_learn.skip_left = g_learn.skip_next;
+        if(g_learn.skip_next < 1024) g_learn.skip_next *= 2;
         return;
       }
     }
   }
   g_learn.flex_iters += ws->sc_host->iter;
+  /* a solve the network carried past one stall window resets the backoff;
+   * shorter ones say nothing about it */
+  if(ws->sc_host->iter >= g_learn.stall_window)
+    g_learn.skip_next = g_learn.skip_after_stall;
 }
 
 /* ---- C ABI of the learned form (no reference counterpart) ---- */
@@ -1366,7 +1393,27 @@ extern "C" real bb_learn_train(int iters, real lr, unsigned long seed)
   BB_CHECK(hipEventDestroy(e1));
   g_learn.train_seconds += 1e-3 * ms;
   g_learn.active = 1;
+  g_learn.skip_left = 0;
+  g_learn.skip_next = g_learn.skip_after_stall;
   return ntail ? tail / ntail : -1.;
+}
+
+/* loss of stored pair k under the current network (no training step) */
+extern "C" real bb_learn_pair_loss(int k)
+{
+  if(k < 0 || k >= g_learn.count || !bb_unet_ready()) return -1.;
+  size_t n = g_learn.n;
+  return bb_unet_train_pair(g_learn.pr + k * n, g_learn.pe + k * n,
+    g_learn.pf + k * n, 0.f, 0, g_bb.stream);
+}
+
+/* |A e - r| / |r| of stored pair k (fluid rows) */
+extern "C" real bb_learn_pair_residual(int k)
+{
+  if(k < 0 || k >= g_learn.count || !bb_unet_ready()) return -1.;
+  size_t n = g_learn.n;
+  return bb_unet_pair_residual(g_learn.pr + k * n, g_learn.pe + k * n,
+    g_learn.pf + k * n, g_bb.stream);
 }
 
 /* use (1) or ignore (0) the network in the learned form */
@@ -1375,15 +1422,19 @@ extern "C" void bb_learn_enable(int on)
   g_learn.active = on && bb_unet_ready();
 }
 
-extern "C" void bb_learn_set_fallback(int window, real drop)
+extern "C" void bb_learn_set_fallback(int window, real drop, int skip)
 {
   if(window > 0 && window <= 60) g_learn.stall_window = window;
   if(drop > 0) g_learn.stall_drop = drop;
+  if(skip >= 0) g_learn.skip_after_stall = skip;
+  g_learn.skip_left = 0;
+  g_learn.skip_next = g_learn.skip_after_stall;
 }
 
 extern "C" void bb_learn_stats(long *applications, long *fallbacks,
-  long *flex_iters, long *jacobi_iters, real *train_seconds)
+  long *flex_iters, long *jacobi_iters, long *skipped, real *train_seconds)
 {
+  if(skipped) *skipped = g_learn.skipped;
   if(applications) *applications = g_learn.applications;
   if(fallbacks) *fallbacks = g_learn.fallbacks;
   if(flex_iters) *flex_iters = g_learn.flex_iters;
@@ -1393,7 +1444,7 @@ extern "C" void bb_learn_stats(long *applications, long *fallbacks,
 
 extern "C" void bb_learn_reset_stats(void)
 {
-  g_learn.applications = g_learn.fallbacks = 0;
+  g_learn.applications = g_learn.fallbacks = g_learn.skipped = 0;
   g_learn.flex_iters = g_learn.jacobi_iters = 0;
 }
 
@@ -1470,9 +1521,14 @@ static void solve_mixed(const PPGeom &G, const DinvTable &tab)
     if(need > inner_tol) inner_tol = need;
     int imax = g_inner_max;
     if(imax > pp_max_iter - total_inner) imax = pp_max_iter - total_inner;
-    if(g_bb.solver_mode == BB_SOLVER_LEARNED && g_learn.active) {
+    if(g_bb.solver_mode == BB_SOLVER_LEARNED && g_learn.active &&
+       g_learn.skip_left <= 0) {
       flexible_run(G, tab, inner_tol, imax);
     } else {
+      if(g_bb.solver_mode == BB_SOLVER_LEARNED && g_learn.active) {
+        g_learn.skip_left--;
+        g_learn.skipped++;
+      }
       bicgstab_run<float>(G, tab, ws->bf, ws->xf, ws->rf, ws->rsf, ws->pf,
         ws->vf, ws->pf2, ws->vf2, ws->tf, inner_tol, imax, 0, g_poll, kinds);
       collect_pair(ws);

@@ -27,5 +27,8 @@ void bb_unet_apply_dev(const float *in, const uint8_t *flags, float *out,
 double bb_unet_train_pair(const float *r, const float *e,
   const uint8_t *flags, float lr, int step, hipStream_t st);
 void bb_unet_free(void);
+/* |A e - r| / |r| over fluid rows (synchronises `st`) */
+double bb_unet_pair_residual(const float *r, const float *e,
+  const uint8_t *flags, hipStream_t st);
 
 #endif

@@ -17,9 +17,11 @@
  *   c2 = conv(x2; 32->16)
  *   u1 = L(up(c2) + x1)  y1 = L(conv(u1; 16->16))  c1 = conv(y1; 16->8)
  *   u0 = L(up(c1) + x0)  y0 = L(conv(u0; 8->8))    o  = conv(y0; 8->1)
- *   M(r) = D^-1 r + h^2 o  on fluid cells,  r  on solid cells
- * (pool: mean of 2x2x2 children; up: nearest parent; D: the operator's
- * diagonal).  The last convolution starts at zero, so an untrained network
+ *   M(r) = D^-1 rt + (h^2/64) o  on fluid cells,  r  on solid cells
+ * where the network's input rt = r - A_fs r_s is r with the solid
+ * unknowns (whose rows are the identity, so e_s = r_s) eliminated from
+ * the fluid rows (pool: mean of 2x2x2 children; up: nearest parent; D:
+ * the operator's diagonal).  The last convolution starts at zero, so an untrained network
  * is exactly the Jacobi preconditioner.  79,920 weights.
  *
  * Layout: activations are BF16, channel-last ((z*ny + y)*nx + x)*C + c,
@@ -173,9 +175,10 @@ struct EpiGate {           /* BF16 out times L'(gate) (backward) */
   }
 };
 
-struct EpiFinal {          /* M(r) = D^-1 r + h^2 o (fluid), r (solid) */
+struct EpiFinal {          /* M(r) = D^-1 rt + s o (fluid), r (solid) */
   float *__restrict__ out;
   const float *__restrict__ r;
+  const float *__restrict__ rt;    /* r with the solid unknowns eliminated */
   const uint8_t *__restrict__ flags;
   UNetDinv dinv;
   float h2;
@@ -185,8 +188,7 @@ struct EpiFinal {          /* M(r) = D^-1 r + h^2 o (fluid), r (solid) */
     if(co != 0) return;
     long cell = ((long) z * ny + y) * nx + x;
     uint8_t f = flags[cell];
-    float rv = r[cell];
-    out[cell] = (f & BB_SOLID) ? rv : dinv.d[f & 127] * rv + h2 * v;
+    out[cell] = (f & BB_SOLID) ? r[cell] : dinv.d[f & 127] * rt[cell] + h2 * v;
   }
 };
 
@@ -545,10 +547,47 @@ k_pool_bwd(int nx, int ny, int nz, const bf16 *__restrict__ gskip,
 
 /* ---------------------------------------------------------------------
  * Training loss: with d = mask (M(r) - e) on fluid cells, q = A_ff d and
- * loss = |q|^2 / |r|^2; dL/dM(r) = 2 A_ff q / |r|^2 (A_ff, the operator
- * between fluid cells, is symmetric).  g_o = h^2 mask dL/dM(r) in BF16.
+ * loss = |q|^2 / |rt|^2 (rt: r with the solid unknowns eliminated);
+ * dL/dM(r) = 2 A_ff q / |rt|^2 (A_ff, the operator between fluid cells, is
+ * symmetric).  g_o = s mask dL/dM(r) in BF16 (s: the output scale).
  * ------------------------------------------------------------------- */
 struct AffGeom { int nx, ny, nz, perx, pery, perz; float ddx, ddy, ddz; };
+
+/* Solid rows of A are the identity, so e = r there exactly; eliminating
+ * them leaves the fluid system A_ff e_f = rt with
+ * rt = r - A_fs r_s on fluid cells (0 on solid ones). */
+__global__ void __launch_bounds__(256)
+k_eliminate(AffGeom G, const uint8_t *__restrict__ fl,
+  const float *__restrict__ r, float *__restrict__ rt)
+{
+  long n = (long) G.nx * G.ny * G.nz;
+  for(long C = (long) blockIdx.x * 256 + threadIdx.x; C < n;
+      C += (long) gridDim.x * 256) {
+    uint8_t f = fl[C];
+    float out = 0.f;
+    if(!(f & BB_SOLID)) {
+      int i = C % G.nx, j = (C / G.nx) % G.ny, k = C / ((long) G.nx * G.ny);
+      auto nb = [&](int ii, int jj, int kk) -> long {
+        if(ii < 0) ii += G.nx; if(ii >= G.nx) ii -= G.nx;
+        if(jj < 0) jj += G.ny; if(jj >= G.ny) jj -= G.ny;
+        if(kk < 0) kk += G.nz; if(kk >= G.nz) kk -= G.nz;
+        return ((long) kk * G.ny + jj) * G.nx + ii;
+      };
+      auto sol = [&](long q) -> float {
+        return (fl[q] & BB_SOLID) ? r[q] : 0.f;
+      };
+      float s = r[C];
+      if(f & BB_FW) s -= G.ddx * sol(nb(i - 1, j, k));
+      if(f & BB_FE) s -= G.ddx * sol(nb(i + 1, j, k));
+      if(f & BB_FS) s -= G.ddy * sol(nb(i, j - 1, k));
+      if(f & BB_FN) s -= G.ddy * sol(nb(i, j + 1, k));
+      if(f & BB_FB) s -= G.ddz * sol(nb(i, j, k - 1));
+      if(f & BB_FT) s -= G.ddz * sol(nb(i, j, k + 1));
+      out = s;
+    }
+    rt[C] = out;
+  }
+}
 
 /* y = A_ff (mask a [- mask b]) on fluid rows, 0 on solid rows */
 __global__ void __launch_bounds__(256)
@@ -664,7 +703,7 @@ struct UNet {
   /* gradients */
   bf16 *go, *gy0, *gp0, *gc1, *gy1, *gp1, *gc2, *gx2, *ga2, *gq1, *gx1,
     *ga1, *gq0, *gx0, *ga0;
-  float *out, *tmp, *tmp2;        /* level-0 FP32 */
+  float *out, *tmp, *tmp2, *rt;   /* level-0 FP32 */
   float *part;                    /* weight-gradient partials */
   long part_cap;
   double *dpart, *dscal;
@@ -716,7 +755,7 @@ void bb_unet_free(void)
     N->x1, N->q1, N->a2, N->x2, N->c2, N->u1, N->y1, N->c1, N->u0, N->y0,
     N->go, N->gy0, N->gp0, N->gc1, N->gy1, N->gp1, N->gc2, N->gx2, N->ga2,
     N->gq1, N->gx1, N->ga1, N->gq0, N->gx0, N->ga0, N->out, N->tmp,
-    N->tmp2, N->part, N->dpart, N->dscal};
+    N->tmp2, N->rt, N->part, N->dpart, N->dscal};
   for(size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree_(ptrs[i]);
   for(int l = 0; l < UNET_NLAYERS; l++) { dfree_(N->wpk[l]); dfree_(N->wpkT[l]); }
   memset(N, 0, sizeof(*N));
@@ -785,6 +824,7 @@ static int unet_ensure(void)
   N->ga0 = dalloc<bf16>(8 * n0);
   N->out = dalloc<float>(n0); N->tmp = dalloc<float>(n0);
   N->tmp2 = dalloc<float>(n0);
+  N->rt = dalloc<float>(n0);
   N->part_cap = 1024L * 27 * 32 * 32;
   N->part = dalloc<float>(N->part_cap);
   N->dpart = dalloc<double>(4096);
@@ -863,6 +903,18 @@ static void pool(const UNet *N, int lc, const bf16 *x, bf16 *q, hipStream_t st)
   BB_CHECK_LAUNCH();
 }
 
+static AffGeom aff_geom(const UNet *N)
+{
+  const dom_struct *d = &dom[0];
+  AffGeom a;
+  a.nx = N->nx[0]; a.ny = N->ny[0]; a.nz = N->nz[0];
+  a.perx = N->per[0]; a.pery = N->per[1]; a.perz = N->per[2];
+  a.ddx = (float) (1. / (d->dx * d->dx));
+  a.ddy = (float) (1. / (d->dy * d->dy));
+  a.ddz = (float) (1. / (d->dz * d->dz));
+  return a;
+}
+
 /* Forward pass on the FP32 vector r.  train = 1 keeps every activation
  * (u0, u1 materialised) for the backward pass; 0 fuses the skip-up into
  * the decoder convolutions' loads. */
@@ -871,8 +923,13 @@ static void unet_forward(UNet *N, const float *r, const uint8_t *flags,
 {
   unet_pack(N, st);
   const ConvGeom G0 = lvl_geom(N, 0), G1 = lvl_geom(N, 1), G2 = lvl_geom(N, 2);
-  conv<1, 8>(G0, InF32{r, N->nx[0], N->ny[0]}, N->wpk[0], st_<8>(N->a0, N, 0, 1), st);
+  k_eliminate<<<elementwise_grid(N->ncell[0]), 256, 0, st>>>(aff_geom(N),
+    flags, r, N->rt);
+  BB_CHECK_LAUNCH();
+  conv<1, 8>(G0, InF32{N->rt, N->nx[0], N->ny[0]}, N->wpk[0], st_<8>(N->a0, N, 0, 1), st);
+  if(!train) bb_prof_begin(BB_K_UNET_CONV0);
   conv<8, 8>(G0, inb<8>(N->a0, N, 0), N->wpk[1], st_<8>(N->x0, N, 0, 1), st);
+  if(!train) bb_prof_end(BB_K_UNET_CONV0);
   pool<8>(N, 1, N->x0, N->q0, st);
   conv<8, 16>(G1, inb<8>(N->q0, N, 1), N->wpk[2], st_<16>(N->a1, N, 1, 1), st);
   conv<16, 16>(G1, inb<16>(N->a1, N, 1), N->wpk[3], st_<16>(N->x1, N, 1, 1), st);
@@ -897,9 +954,11 @@ static void unet_forward(UNet *N, const float *r, const uint8_t *flags,
     conv<8, 8>(G0, inb<8>(N->u0, N, 0), N->wpk[9], st_<8>(N->y0, N, 0, 1), st);
   } else {
     InSkipUp<8> su{N->x0, N->c1, N->nx[0], N->ny[0], N->nx[1], N->ny[1]};
+    bb_prof_begin(BB_K_UNET_CONV0);
     conv<8, 8>(G0, su, N->wpk[9], st_<8>(N->y0, N, 0, 1), st);
+    bb_prof_end(BB_K_UNET_CONV0);
   }
-  EpiFinal fin{out, r, flags, dinv, h2, N->nx[0], N->ny[0]};
+  EpiFinal fin{out, r, N->rt, flags, dinv, h2, N->nx[0], N->ny[0]};
   conv<8, 1>(G0, inb<8>(N->y0, N, 0), N->wpk[10], fin, st);
 }
 
@@ -968,21 +1027,10 @@ static void unet_backward(UNet *N, const float *r, hipStream_t st)
   /* x0 = L(conv(a0; W1)) */
   dw<8, 8>(N, 1, G0, inb<8>(N->a0, N, 0), N->gx0, st);
   conv<8, 8>(G0, inb<8>(N->gx0, N, 0), N->wpkT[1], gt_<8>(N->ga0, N->a0, N, 0), st);
-  /* a0 = L(conv(r; W0)) */
-  dw<1, 8>(N, 0, G0, InF32{r, N->nx[0], N->ny[0]}, N->ga0, st);
+  /* a0 = L(conv(rt; W0)) */
+  dw<1, 8>(N, 0, G0, InF32{N->rt, N->nx[0], N->ny[0]}, N->ga0, st);
 }
 
-static AffGeom aff_geom(const UNet *N)
-{
-  const dom_struct *d = &dom[0];
-  AffGeom a;
-  a.nx = N->nx[0]; a.ny = N->ny[0]; a.nz = N->nz[0];
-  a.perx = N->per[0]; a.pery = N->per[1]; a.perz = N->per[2];
-  a.ddx = (float) (1. / (d->dx * d->dx));
-  a.ddy = (float) (1. / (d->dy * d->dy));
-  a.ddz = (float) (1. / (d->dz * d->dz));
-  return a;
-}
 
 /* ---------------------------------------------------------------------
  * Internal API (bb_unet.h) used by the flexible solver.
@@ -1005,12 +1053,20 @@ UNetDinv bb_unet_dinv(void)
   return t;
 }
 
+/* The network's output is scaled by h^2 / 64: Adam moves every weight by
+ * about its learning rate per step, and the training loss weighs the
+ * output with A twice (about (12/h^2)^2), so an unscaled last layer makes
+ * the first steps overshoot by orders of magnitude. */
+#define UNET_OUT_SCALE (1. / 64.)
+
 float bb_unet_h2(void)
 {
   const dom_struct *d = &dom[0];
-  return (float) (3. / (1. / (d->dx * d->dx) + 1. / (d->dy * d->dy)
-    + 1. / (d->dz * d->dz)));
+  return (float) (UNET_OUT_SCALE * 3. / (1. / (d->dx * d->dx)
+    + 1. / (d->dy * d->dy) + 1. / (d->dz * d->dz)));
 }
+
+extern "C" real bb_unet_output_scale(void) { return bb_unet_h2(); }
 
 int bb_unet_ready(void) { return unet_ensure(); }
 
@@ -1032,8 +1088,8 @@ static void unet_loss_grad(UNet *N, const float *r, const float *e,
   const long n0 = N->ncell[0];
   const int eb = elementwise_grid(n0);
   AffGeom ag = aff_geom(N);
-  /* |r|^2 */
-  k_dot_f32<<<eb, 256, 0, st>>>(n0, r, N->dpart);
+  /* |rt|^2, the fluid system's right-hand side */
+  k_dot_f32<<<eb, 256, 0, st>>>(n0, N->rt, N->dpart);
   BB_CHECK_LAUNCH();
   k_sum_parts<<<1, 256, 0, st>>>(eb, N->dpart, N->dscal);
   BB_CHECK_LAUNCH();
@@ -1074,6 +1130,47 @@ double bb_unet_train_pair(const float *r, const float *e, const uint8_t *flags,
     hipMemcpyDeviceToHost, st));
   BB_CHECK(hipStreamSynchronize(st));
   return h[0] > 0. ? h[1] / h[0] : h[1];
+}
+
+__global__ void __launch_bounds__(256)
+k_resid_fluid(long n, const uint8_t *__restrict__ fl,
+  const float *__restrict__ y, const float *__restrict__ r,
+  double *__restrict__ part)
+{
+  __shared__ double sh[4];
+  double acc = 0.;
+  for(long C = (long) blockIdx.x * 256 + threadIdx.x; C < n;
+      C += (long) gridDim.x * 256)
+    if(!(fl[C] & BB_SOLID)) { double d = (double) y[C] - r[C]; acc += d * d; }
+  double t = bb_block_sum(acc, sh);
+  if(threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+/* |A_ff e - rt| / |rt| over fluid rows, for checking a stored pair */
+double bb_unet_pair_residual(const float *r, const float *e,
+  const uint8_t *flags, hipStream_t st)
+{
+  UNet *N = &g_net;
+  const long n0 = N->ncell[0];
+  const int eb = elementwise_grid(n0);
+  k_eliminate<<<eb, 256, 0, st>>>(aff_geom(N), flags, r, N->rt);
+  BB_CHECK_LAUNCH();
+  k_aff<<<eb, 256, 0, st>>>(aff_geom(N), flags, bb_unet_dinv(), e, NULL,
+    N->tmp, NULL);
+  BB_CHECK_LAUNCH();
+  k_resid_fluid<<<eb, 256, 0, st>>>(n0, flags, N->tmp, N->rt, N->dpart);
+  BB_CHECK_LAUNCH();
+  k_sum_parts<<<1, 256, 0, st>>>(eb, N->dpart, N->dscal + 2);
+  BB_CHECK_LAUNCH();
+  k_dot_f32<<<eb, 256, 0, st>>>(n0, N->rt, N->dpart);
+  BB_CHECK_LAUNCH();
+  k_sum_parts<<<1, 256, 0, st>>>(eb, N->dpart, N->dscal + 3);
+  BB_CHECK_LAUNCH();
+  double h[2];
+  BB_CHECK(hipMemcpyAsync(h, N->dscal + 2, 2 * sizeof(double),
+    hipMemcpyDeviceToHost, st));
+  BB_CHECK(hipStreamSynchronize(st));
+  return h[1] > 0. ? sqrt(h[0] / h[1]) : sqrt(h[0]);
 }
 
 /* ---------------------------------------------------------------------

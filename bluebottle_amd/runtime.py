@@ -77,6 +77,18 @@ EXTRA_HIP = {
     "bb_set_poll_interval": (None, [C.c_int]),
     "bb_set_mixed_params": (None, [C.c_double, C.c_int, C.c_int]),
     "bb_set_implicit": (None, [C.c_int]),
+    "bb_learn_collect": (None, [C.c_int]),
+    "bb_learn_pairs": (C.c_int, []),
+    "bb_learn_train": (C.c_double, [C.c_int, C.c_double, C.c_ulong]),
+    "bb_learn_enable": (None, [C.c_int]),
+    "bb_learn_pair_loss": (C.c_double, [C.c_int]),
+    "bb_learn_set_fallback": (None, [C.c_int, C.c_double, C.c_int]),
+    "bb_learn_stats": (None, [C.POINTER(C.c_long)] * 5
+                       + [C.POINTER(C.c_double)]),
+    "bb_learn_reset_stats": (None, []),
+    "bb_unet_num_params": (C.c_long, []),
+    "bb_unet_init": (None, [C.c_ulong]),
+    "bb_unet_apply": (None, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "bb_is_implicit": (C.c_int, []),
     "bb_copy_bandwidth": (C.c_double, [C.c_long, C.c_int]),
     "bb_device_alloc": (C.c_void_p, [C.c_size_t]),
@@ -90,7 +102,8 @@ PROF_KINDS = [
     "u_star", "bc", "pp_setup", "pack", "spmv_p", "spmv_s", "update_xr",
     "update_p", "final", "resid64", "correct64", "spmv_p32", "spmv_s32",
     "update_xr32", "update_p32", "project", "update_pressure", "find_dt",
-    "cages", "part_bc", "quadrature", "move_parts", "misc",
+    "cages", "part_bc", "quadrature", "move_parts", "misc", "unet",
+    "unet_conv0", "fspmv", "fvec",
 ]
 
 

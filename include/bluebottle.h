@@ -396,8 +396,10 @@ void bb_set_mixed_params(real inner_tol, int inner_max, int outer_max);
 /* Per-kernel event timing on the library stream.  Kinds index the kernel
  * families (predictor, BCs, Poisson set-up, the four BiCGSTAB sweeps in
  * FP64 and FP32, refinement residual and correction, projection, pressure
- * update, dt, cages, particle BCs, quadrature, particle motion, misc), in
- * that order, see BB_PROF_* below. */
+ * update, dt, cages, particle BCs, quadrature, particle motion, misc; the
+ * learned form's network application, its full-resolution 8->8
+ * convolutions, its two products with A and its vector updates), in that
+ * order, see BB_PROF_* below. */
 enum {
   BB_PROF_USTAR = 0, BB_PROF_BC, BB_PROF_PP_SETUP, BB_PROF_PACK,
   BB_PROF_SPMV_P, BB_PROF_SPMV_S, BB_PROF_UPDATE_XR, BB_PROF_UPDATE_P,
@@ -405,6 +407,7 @@ enum {
   BB_PROF_SPMV_S32, BB_PROF_UPDATE_XR32, BB_PROF_UPDATE_P32,
   BB_PROF_PROJECT, BB_PROF_UPDATE_PRESSURE, BB_PROF_FIND_DT, BB_PROF_CAGES,
   BB_PROF_PART_BC, BB_PROF_QUADRATURE, BB_PROF_MOVE_PARTS, BB_PROF_MISC,
+  BB_PROF_UNET, BB_PROF_UNET_CONV0, BB_PROF_FSPMV, BB_PROF_FVEC,
   BB_PROF_NKINDS
 };
 void bb_prof_enable(int on);
@@ -442,14 +445,24 @@ int bb_learn_pairs(void);
  * train on, or grid sizes not divisible by 4).  Enables the network. */
 real bb_learn_train(int iters, real lr, unsigned long seed);
 void bb_learn_enable(int on);
+/* the loss of stored pair k under the current network (-1: no such pair) */
+real bb_learn_pair_loss(int k);
 /* fallback test: every `window` iterations, |r| must fall below `drop`
- * times its value `window` iterations earlier (defaults 5 and 0.9) */
-void bb_learn_set_fallback(int window, real drop);
+ * times its value `window` iterations earlier (defaults 5 and 0.9); after
+ * a fallback the next `skip` inner solves run Jacobi BiCGSTAB directly,
+ * `skip` doubling with each fallback up to 1024 and returning to its start
+ * after a retraining or a solve of at least `window` iterations that the
+ * network finishes
+ * (default 8; values < 0 leave a setting unchanged) */
+void bb_learn_set_fallback(int window, real drop, int skip);
 void bb_learn_stats(long *applications, long *fallbacks, long *flex_iters,
-  long *jacobi_iters, real *train_seconds);
+  long *jacobi_iters, long *skipped, real *train_seconds);
 void bb_learn_reset_stats(void);
 /* network weights (FP32 masters, layer by layer, each [cout][cin][27]) */
 long bb_unet_num_params(void);
+/* the factor of the network's output in M(r) = D^-1 r + factor * net(r) on
+ * fluid cells (h^2 / 64 for the current domain) */
+real bb_unet_output_scale(void);
 void bb_unet_init(unsigned long seed);
 void bb_unet_set_params(const float *host, long n);
 void bb_unet_get_params(float *host, long n);

@@ -24,8 +24,11 @@ void bb_unet_apply_dev(const float *, const uint8_t *, float *, hipStream_t)
 double bb_unet_train_pair(const float *, const float *, const uint8_t *,
   float, int, hipStream_t) { not_emulated("bb_unet_train_pair"); return 0.; }
 void bb_unet_free(void) {}
+double bb_unet_pair_residual(const float *, const float *, const uint8_t *,
+  hipStream_t) { not_emulated("bb_unet_pair_residual"); return 0.; }
 extern "C" {
 long bb_unet_num_params(void) { return 79920; }
+real bb_unet_output_scale(void) { return 0.; }
 void bb_unet_init(unsigned long) { not_emulated("bb_unet_init"); }
 void bb_unet_set_params(const float *, long) { not_emulated("bb_unet_set_params"); }
 void bb_unet_get_params(float *, long) { not_emulated("bb_unet_get_params"); }

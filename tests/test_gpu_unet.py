@@ -7,8 +7,9 @@
 * the weight gradients of the training loss against PyTorch autograd
   through the same BF16-rounded forward pass (the GPU's backward runs in
   BF16 with FP32 accumulation);
-* an untrained network (last layer zero) is exactly the Jacobi
-  preconditioner; Adam steps lower the loss on a fixed pair;
+* an untrained network (last layer zero) is the Jacobi preconditioner of
+  the fluid system left when the solid unknowns (identity rows) are
+  eliminated; Adam steps lower the loss on a fixed pair;
 * the learned solver form reaches the same true FP64 residual as the
   others, matches the oracle's phi within the mixed form's tolerance, and
   a garbage network trips the fallback to Jacobi and still converges.
@@ -54,9 +55,9 @@ def _bind(hip):
     hip.bb_learn_train.argtypes = [C.c_int, C.c_double, C.c_ulong]
     hip.bb_learn_train.restype = C.c_double
     hip.bb_learn_enable.argtypes = [C.c_int]
-    hip.bb_learn_set_fallback.argtypes = [C.c_int, C.c_double]
+    hip.bb_learn_set_fallback.argtypes = [C.c_int, C.c_double, C.c_int]
     L = C.POINTER(C.c_long)
-    hip.bb_learn_stats.argtypes = [L, L, L, L, C.POINTER(C.c_double)]
+    hip.bb_learn_stats.argtypes = [L, L, L, L, L, C.POINTER(C.c_double)]
 
 
 class _Dev:
@@ -119,7 +120,10 @@ def _setup(which):
            int(case.bc_p[2][0] == abi.PERIODIC),
            int(case.bc_p[4][0] == abi.PERIODIC))
     shape = (d.zn, d.yn, d.xn)
-    h2 = np.float32(3.0 / (1 / d.dx ** 2 + 1 / d.dy ** 2 + 1 / d.dz ** 2))
+    s.hip.bb_unet_output_scale.restype = C.c_double
+    h2 = np.float32(s.hip.bb_unet_output_scale())
+    assert abs(h2 / (3.0 / (1 / d.dx ** 2 + 1 / d.dy ** 2 + 1 / d.dz ** 2)
+                     / 64) - 1) < 1e-6
     dinv, diag = ur.dinv_tables(d.dx, d.dy, d.dz)
     dd = (np.float32(1 / d.dx ** 2), np.float32(1 / d.dy ** 2),
           np.float32(1 / d.dz ** 2))
@@ -154,11 +158,12 @@ def test_forward_matches_torch(which):
     finally:
         dev.free()
     ref = ur.forward(torch.from_numpy(r), torch.from_numpy(w), fl, h2, dinv,
-                     per).numpy()
+                     per, dd).numpy()
+    rt = ur.eliminate(torch.from_numpy(r), fl, dd).numpy()
     solid = (fl & ur.SOLID) != 0
     assert np.array_equal(got[solid], r[solid])
-    net = got - np.where(solid, r, dinv[fl & 127] * r)
-    net_ref = ref - np.where(solid, r, dinv[fl & 127] * r)
+    net = got - np.where(solid, r, dinv[fl & 127] * rt)
+    net_ref = ref - np.where(solid, r, dinv[fl & 127] * rt)
     err = np.max(np.abs(net - net_ref)) / np.max(np.abs(net_ref))
     print("unet forward %s: rel err %.2e (scale %.2e)" % (
         which, err, np.max(np.abs(net_ref))), flush=True)
@@ -181,9 +186,11 @@ def test_untrained_network_is_jacobi():
         got = dev.get(od, r)
     finally:
         dev.free()
-    want = np.where((fl & ur.SOLID) != 0, r, dinv[fl & 127] * r)
+    rt = ur.eliminate(torch.from_numpy(r), fl, dd).numpy()
+    want = np.where((fl & ur.SOLID) != 0, r, dinv[fl & 127] * rt)
     s.close()
-    assert np.array_equal(got, want.astype(np.float32))
+    # D^-1 rt, with rt formed in the kernel's operation order
+    assert np.allclose(got, want, rtol=1e-6, atol=0)
 
 
 @pytest.mark.parametrize("which", ["periodic", "channel"])
@@ -202,7 +209,7 @@ def test_gradients_match_torch(which):
     # which amplifies rounding noise)
     wp = (w * (1 + 0.1 * rng.standard_normal(w.size))).astype(np.float32)
     e = ur.forward(torch.from_numpy(r), torch.from_numpy(wp), fl, h2, dinv,
-                   per).numpy()
+                   per, dd).numpy()
     dev = _Dev(hip)
     try:
         rd, ed, fd = dev.put(r), dev.put(e), dev.put(fl)
@@ -251,7 +258,7 @@ def test_adam_lowers_the_loss():
     fl = _flags(shape, per, 23)
     wt = (w0 * (1 + 0.3 * rng.standard_normal(w0.size))).astype(np.float32)
     e = ur.forward(torch.from_numpy(r), torch.from_numpy(wt), fl, h2, dinv,
-                   per).numpy()
+                   per, dd).numpy()
     dev = _Dev(hip)
     try:
         rd, ed, fd = dev.put(r), dev.put(e), dev.put(fl)
@@ -296,6 +303,7 @@ def test_learned_form_solves_like_the_oracle(garbage):
     else:
         hip.bb_unet_init(32)
     hip.bb_learn_enable(1)
+    hip.bb_learn_set_fallback(5, 0.9, 0)   # every inner solve tries the network
     hip.bb_learn_reset_stats()
     s.set_solver(2)
     try:
@@ -308,9 +316,9 @@ def test_learned_form_solves_like_the_oracle(garbage):
     s.pull()
     st = s.solve_stats()
     a, fb, fi, ji = C.c_long(), C.c_long(), C.c_long(), C.c_long()
-    ts = C.c_double()
+    sk, ts = C.c_long(), C.c_double()
     hip.bb_learn_stats(C.byref(a), C.byref(fb), C.byref(fi), C.byref(ji),
-                       C.byref(ts))
+                       C.byref(sk), C.byref(ts))
     print("learned solve (garbage=%s): outer %d inner %d resid %.2e; net "
           "applications %d, fallbacks %d, flexible its %d, Jacobi its %d" % (
               garbage, st.outer_steps, st.iterations, st.residual, a.value,

@@ -79,12 +79,28 @@ def dinv_tables(dx, dy, dz):
     return d, g
 
 
-def forward(r, w, flags, h2, dinv, per, bf16=True):
+def eliminate(r, flags, dd):
+    """rt = r - A_fs r_s on fluid cells, 0 on solid cells (the solid rows
+    of A are the identity)."""
+    fl = torch.from_numpy(flags.astype(np.int64))
+    solid = (fl & SOLID) != 0
+    rs = torch.where(solid, r, torch.zeros_like(r))
+    out = r.clone()
+    for bit, axis, shift, d in ((FW, 2, 1, dd[0]), (FE, 2, -1, dd[0]),
+                                (FS, 1, 1, dd[1]), (FN, 1, -1, dd[1]),
+                                (FB, 0, 1, dd[2]), (FT, 0, -1, dd[2])):
+        nb = torch.roll(rs, shifts=shift, dims=axis)
+        out = out - torch.where((fl & bit) != 0, d * nb, torch.zeros_like(nb))
+    return torch.where(solid, torch.zeros_like(out), out)
+
+
+def forward(r, w, flags, h2, dinv, per, dd, bf16=True):
     """r (nz, ny, nx) float32 tensor; w flat tensor; flags uint8 array of the
-    same shape; returns M(r) (nz, ny, nx)."""
+    same shape; dd = (1/dx^2, 1/dy^2, 1/dz^2); returns M(r) (nz, ny, nx)."""
     Ws = split(w)
     b = bf16
-    x = _bf(r[None], b)
+    rt = eliminate(r, flags, dd)
+    x = _bf(rt[None], b)
     a0 = _bf(_act(_conv(x, Ws[0], per, b)), b)
     x0 = _bf(_act(_conv(a0, Ws[1], per, b)), b)
     q0 = _bf(_pool(x0), b)
@@ -103,7 +119,7 @@ def forward(r, w, flags, h2, dinv, per, bf16=True):
     fl = torch.from_numpy(flags.astype(np.int64))
     solid = (fl & SOLID) != 0
     d = torch.from_numpy(dinv)[fl & 127]
-    return torch.where(solid, r, d * r + h2 * o)
+    return torch.where(solid, r, d * rt + h2 * o)
 
 
 def a_ff(y, flags, ddx, ddy, ddz, diag):
@@ -122,9 +138,10 @@ def a_ff(y, flags, ddx, ddy, ddz, diag):
 
 
 def loss(r, e, w, flags, h2, dinv, diag, per, dd, bf16=False):
-    out = forward(r, w, flags, h2, dinv, per, bf16=bf16)
+    out = forward(r, w, flags, h2, dinv, per, dd, bf16=bf16)
     fl = torch.from_numpy(flags.astype(np.int64))
     fluid = (fl & SOLID) == 0
     d = torch.where(fluid, out - e, torch.zeros_like(out))
     q = a_ff(d, flags, dd[0], dd[1], dd[2], diag)
-    return (q * q).sum() / (r * r).sum()
+    rt = eliminate(r, flags, dd)
+    return (q * q).sum() / (rt * rt).sum()
