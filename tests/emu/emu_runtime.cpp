@@ -125,6 +125,22 @@ T shfl(std::vector<T> &buf, T v, unsigned delta, int width)
   return out;
 }
 
+template<typename T>
+T shfl_idx(std::vector<T> &buf, T v, int src, int width)
+{
+  int tid = flat_tid();
+  buf[tid] = v;
+  g_blk.bar->arrive_and_wait();
+  int lane = tid % width;
+  int base = tid - lane;
+  int s = ((src % width) + width) % width;
+  T out = v;
+  if(base + s < g_blk.nthreads) out = buf[base + s];
+  g_blk.bar->arrive_and_wait();
+  (void) lane;
+  return out;
+}
+
 }  // namespace
 
 namespace bb_emu {
@@ -134,6 +150,7 @@ void syncthreads(void) { g_blk.bar->arrive_and_wait(); }
 double shfl_down_d(double v, unsigned d, int w) { return shfl(g_blk.xd, v, d, w); }
 float shfl_down_f(float v, unsigned d, int w) { return shfl(g_blk.xf, v, d, w); }
 int shfl_down_i(int v, unsigned d, int w) { return shfl(g_blk.xi, v, d, w); }
+double shfl_idx_d(double v, int src, int w) { return shfl_idx(g_blk.xd, v, src, w); }
 
 bool is_device_range(const void *p, size_t bytes)
 {

@@ -114,6 +114,7 @@ void syncthreads(void);
 double shfl_down_d(double v, unsigned delta, int width);
 float shfl_down_f(float v, unsigned delta, int width);
 int shfl_down_i(int v, unsigned delta, int width);
+double shfl_idx_d(double v, int src, int width);
 /* true when [p, p+bytes) lies inside one device (or pinned host)
  * allocation; null passes */
 bool is_device_range(const void *p, size_t bytes);
@@ -133,6 +134,7 @@ void isa_capture(const char *name, const char *ftype, dim3 g, dim3 b,
 inline double __shfl_down(double v, unsigned d, int w = 64) { return bb_emu::shfl_down_d(v, d, w); }
 inline float __shfl_down(float v, unsigned d, int w = 64) { return bb_emu::shfl_down_f(v, d, w); }
 inline int __shfl_down(int v, unsigned d, int w = 64) { return bb_emu::shfl_down_i(v, d, w); }
+inline double __shfl(double v, int src, int w = 64) { return bb_emu::shfl_idx_d(v, src, w); }
 
 inline int atomicAdd(int *a, int v) { return __atomic_fetch_add(a, v, __ATOMIC_SEQ_CST); }
 inline int atomicExch(int *a, int v) { return __atomic_exchange_n(a, v, __ATOMIC_SEQ_CST); }
