@@ -1,4 +1,12 @@
-"""Synthetic cases: the five benchmark configurations and small test cases.
+"""Synthetic cases: the benchmark configurations and small test cases, and
+their sim/input files (flow.config, part.config) in the reference's
+grammar.
+
+Configurations: 1 `config1` (48^3 Taylor-Green box, one fixed sphere),
+2 `config2` (128^3, 64 settling spheres), 3 `config3` (256^3, 1000
+settling spheres), 4 the learned preconditioner on configuration 3's flow
+(no case of its own), 5 `config5` (384 x 192 x 384 wall-bounded shear box,
+8000 spheres).
 
 Physical scales (the configurations state Re but not nu or |g|):
   * Taylor-Green cases: unit periodic box, U = L = 1, nu = 1/Re; the
@@ -152,6 +160,34 @@ def wide_case(n=(72, 32, 16), nparts=2, seed=9):
                 gradP=(0.0, 0.0, 0.0, 0.0, -0.05, 0.0))
 
 
+def config5(nx=384, ny=192, nz=384, nparts=8000, lamb_max_iter=25,
+              pp_residual=1e-8):
+    """Dense sheared channel: x in [0, 2], y in [0, 1], z in [0, 2] with
+    cubic cells (384 x 192 x 384), no-slip y walls moving at -U/2 and +U/2
+    (U = 1), p Neumann at the walls, periodic x and z; Re_bulk = U H / nu
+    = 2000 (H = 1, the wall distance); 8000 spheres of radius 4 dx at seeded
+    random positions at least 1 dx apart and 1 dx from the walls (volume
+    fraction 7.6 %), density ratio 1.5, no gravity; L = 4; the initial
+    flow is the linear shear between the walls (init_cond SHEAR)."""
+    dx = 1.0 / ny
+    lo, hi = (0.0, 0.0, 0.0), (nx * dx, ny * dx, nz * dx)
+    a = 4.0 * dx
+    uw = 0.5
+    D, N, P = abi.DIRICHLET, abi.NEUMANN, abi.PERIODIC
+    pos = place_spheres(nparts, a, lo, hi, dx, 5, periodic=(1, 0, 1),
+                        wall_gap=dx)
+    parts = [Particle(x=float(p[0]), y=float(p[1]), z=float(p[2]), r=a,
+                      rho=1.5, order=4) for p in pos]
+    return Case(n=(nx, ny, nz), lo=lo, hi=hi, nu=1.0 / 2000.0, CFL=0.3,
+                pp_max_iter=20000, pp_residual=pp_residual,
+                lamb_max_iter=lamb_max_iter,
+                bc_p=((P, 0.), (P, 0.), (N, 0.), (N, 0.), (P, 0.), (P, 0.)),
+                bc_u=((P, 0.), (P, 0.), (D, -uw), (D, uw), (P, 0.), (P, 0.)),
+                bc_v=((P, 0.), (P, 0.), (D, 0.), (D, 0.), (P, 0.), (P, 0.)),
+                bc_w=((P, 0.), (P, 0.), (D, 0.), (D, 0.), (P, 0.), (P, 0.)),
+                out_plane=abi.NORTH, init="shear", seed=5, particles=parts)
+
+
 def channel_case(n=(16, 12, 16), nparts=0, seed=5):
     """Wall-bounded shear box: no-slip y walls (Dirichlet u = +-U_w), p
     Neumann, periodic x and z."""
@@ -173,3 +209,152 @@ def channel_case(n=(16, 12, 16), nparts=0, seed=5):
                 pp_residual=1e-11, lamb_max_iter=2, bc_p=bc_p, bc_u=bc_u,
                 bc_v=bc_v, bc_w=bc_w, out_plane=abi.NORTH,
                 init="taylor_green", noise=1e-3, seed=seed, particles=parts)
+
+
+# ---------------------------------------------------------------------------
+# sim/input files (the reference's grammar: src/domain.c:30-730,
+# src/particle.c:79-159)
+# ---------------------------------------------------------------------------
+_FACES = "WESNBT"
+_BC_NAME = {abi.PERIODIC: "PERIODIC", abi.DIRICHLET: "DIRICHLET",
+            abi.NEUMANN: "NEUMANN", abi.PRECURSOR: "PRECURSOR"}
+_INIT_NAME = {"quiescent": "QUIESCENT", "shear": "SHEAR",
+              "taylor_green": "TAYLOR_GREEN"}
+_INIT_CODE = {abi.QUIESCENT: "quiescent", abi.SHEAR: "shear",
+              100: "taylor_green"}
+_PLANE_NAME = {abi.WEST: "WEST", abi.EAST: "EAST", abi.SOUTH: "SOUTH",
+               abi.NORTH: "NORTH", abi.BOTTOM: "BOTTOM", abi.TOP: "TOP",
+               abi.HOMOGENEOUS: "HOMOGENEOUS"}
+
+
+def _g(x):
+    return "%.17g" % x
+
+
+def write_case(case, root):
+    """Write root/input/flow.config and root/input/part.config for `case`.
+    The Taylor-Green noise amplitude and seed have no place in the grammar
+    (init_cond TAYLOR_GREEN is this build's extension)."""
+    import os
+    os.makedirs(os.path.join(root, "input"), exist_ok=True)
+    c = case
+    L = []
+    L.append("DOMAIN")
+    ext = []
+    for ax, name in enumerate("XYZ"):
+        ext.append("(%ss, %se, %sn) %s %s %d" % (name, name, name, _g(c.lo[ax]),
+                                                 _g(c.hi[ax]), c.n[ax]))
+    L += ext + ["", "GPU DOMAIN DECOMPOSITION", "DEV RANGE 0 0", "n 1"] + ext
+    L += ["E -1 W -1 N -1 S -1 T -1 B -1", "", "PHYSICAL PARAMETERS",
+          "rho_f " + _g(c.rho_f), "nu " + _g(c.nu), "", "SIMULATION PARAMETERS",
+          "duration " + _g(c.duration), "CFL " + _g(c.CFL),
+          "pp_max_iter %d" % c.pp_max_iter,
+          "pp_residual " + _g(c.pp_residual),
+          "lamb_max_iter %d" % c.lamb_max_iter,
+          "lamb_residual " + _g(c.lamb_residual),
+          "lamb_relax " + _g(c.lamb_relax), "lamb_cut " + _g(c.lamb_cut), "",
+          "BOUNDARY CONDITIONS", "vel_tDelay " + _g(c.vel_tDelay)]
+    for title, comp, spec in (("PRESSURE", "p", c.bc_p),
+                              ("X-VELOCITY", "u", c.bc_u),
+                              ("Y-VELOCITY", "v", c.bc_v),
+                              ("Z-VELOCITY", "w", c.bc_w)):
+        L.append(title)
+        for f, face in enumerate(_FACES):
+            t, val = spec[f]
+            line = "bc.%s%s %s" % (comp, face, _BC_NAME[t])
+            if comp == "p" and t == abi.NEUMANN:
+                line += " 0"
+            elif comp != "p" and t in (abi.DIRICHLET, abi.PRECURSOR):
+                line += " %s 0" % _g(val)
+            L.append(line)
+    gp, gg = c.gradP, c.g
+    L += ["", "INITIAL CONDITION", "init_cond " + _INIT_NAME[c.init], "",
+          "SOLVABILITY ENFORCEMENT PLANE",
+          "out_plane " + _PLANE_NAME[c.out_plane], "",
+          "SIMULATION DRIVING CONDITIONS", "p_tDelay " + _g(c.p_tDelay),
+          "gradP.x %s %s" % (_g(gp[0]), _g(gp[1])),
+          "gradP.y %s %s" % (_g(gp[2]), _g(gp[3])),
+          "gradP.z %s %s" % (_g(gp[4]), _g(gp[5])),
+          "g_tDelay " + _g(c.g_tDelay),
+          "g.x %s %s" % (_g(gg[0]), _g(gg[1])),
+          "g.y %s %s" % (_g(gg[2]), _g(gg[3])),
+          "g.z %s %s" % (_g(gg[4]), _g(gg[5])), "",
+          "PID CONTROLLER GAINS", "Kp 0", "Ki 0", "Kd 0"]
+    with open(os.path.join(root, "input", "flow.config"), "w") as f:
+        f.write("\n".join(L) + "\n")
+    P = ["n %d" % len(c.particles), "(l/a) " + _g(c.interaction_length_ratio)]
+    for p in c.particles:
+        P += ["", "r " + _g(p.r),
+              "(x, y, z) %s %s %s" % (_g(p.x), _g(p.y), _g(p.z)),
+              "(aFx, aFy, aFz) %s %s %s" % tuple(_g(v) for v in p.aF),
+              "(aLx, aLy, aLz) %s %s %s" % tuple(_g(v) for v in p.aL),
+              "rho " + _g(p.rho), "E " + _g(p.E), "sigma " + _g(p.sigma),
+              "e_dry " + _g(p.e_dry), "coeff_fric " + _g(p.coeff_fric),
+              "order %d" % p.order, "rs/r " + _g(p.rs_r),
+              "spring_k " + _g(p.spring_k),
+              "spring (x, y, z) %s %s %s" % (_g(p.spring_x), _g(p.spring_y),
+                                             _g(p.spring_z)),
+              "spring_l " + _g(p.spring_l),
+              "translating %d" % p.translating, "rotating %d" % p.rotating]
+    with open(os.path.join(root, "input", "part.config"), "w") as f:
+        f.write("\n".join(P) + "\n")
+
+
+def read_case(root, noise=1e-3, seed=1):
+    """A Case from root/input/flow.config and root/input/part.config,
+    parsed by the driver library's readers (the reference's grammar)."""
+    import ctypes as C
+    import os
+    from . import runtime
+    host, _ = runtime.load()
+    for name, fn in (("flow.config", host.bb_read_flow_config),
+                     ("part.config", host.bb_read_part_config)):
+        path = os.path.join(root, "input", name).encode()
+        if fn(path) != 0:
+            raise ValueError("could not read %s" % path.decode())
+    D = runtime.gvar("Dom", abi.dom_struct)
+    b = runtime.bc()
+
+    def spec(comp):
+        out = []
+        for face in _FACES:
+            t = getattr(b, comp + face)
+            v = 0.0 if comp == "p" else getattr(b, comp + face + "Dm")
+            out.append((t, v))
+        return tuple(out)
+
+    gs = runtime.get_scalar
+    gp = runtime.gvar("gradP", abi.gradP_struct)
+    gg = runtime.gvar("g", abi.g_struct)
+    init = _INIT_CODE.get(runtime.get_scalar("init_cond", "i"))
+    if init is None:
+        raise ValueError("init_cond %d is not set up here"
+                         % runtime.get_scalar("init_cond", "i"))
+    n = gs("nparts", "i")
+    ptr = C.POINTER(abi.part_struct).in_dll(host, "parts")
+    parts = []
+    for i in range(n):
+        q = ptr[i]
+        parts.append(Particle(
+            x=q.x, y=q.y, z=q.z, r=q.r, rho=q.rho, E=q.E, sigma=q.sigma,
+            e_dry=q.e_dry, coeff_fric=q.coeff_fric, order=q.order,
+            rs_r=q.rs, spring_k=q.spring_k, spring_x=q.spring_x,
+            spring_y=q.spring_y, spring_z=q.spring_z, spring_l=q.spring_l,
+            translating=q.translating, rotating=q.rotating,
+            aF=(q.aFx, q.aFy, q.aFz), aL=(q.aLx, q.aLy, q.aLz)))
+    return Case(n=(D.xn, D.yn, D.zn), lo=(D.xs, D.ys, D.zs),
+                hi=(D.xe, D.ye, D.ze), rho_f=gs("rho_f"), nu=gs("nu"),
+                CFL=gs("CFL"), pp_max_iter=gs("pp_max_iter", "i"),
+                pp_residual=gs("pp_residual"),
+                lamb_max_iter=gs("lamb_max_iter", "i"),
+                lamb_residual=gs("lamb_residual"),
+                lamb_relax=gs("lamb_relax"), lamb_cut=gs("lamb_cut"),
+                out_plane=gs("out_plane", "i"), bc_p=spec("p"),
+                bc_u=spec("u"), bc_v=spec("v"), bc_w=spec("w"),
+                gradP=(gp.xm, gp.xa, gp.ym, gp.ya, gp.zm, gp.za),
+                g=(gg.xm, gg.xa, gg.ym, gg.ya, gg.zm, gg.za),
+                p_tDelay=gs("p_tDelay"), g_tDelay=gs("g_tDelay"),
+                vel_tDelay=gs("vel_tDelay"), init=init, noise=noise,
+                seed=seed, duration=gs("duration"),
+                interaction_length_ratio=gs("interactionLengthRatio"),
+                particles=parts)

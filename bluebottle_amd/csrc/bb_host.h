@@ -38,6 +38,25 @@ int bb_host_prepare(void);
 /* src/bluebottle.c:598-711; returns the Lamb sub-iteration count. */
 int bb_host_step(void);
 
+/* ---- files in the reference's formats (bb_io.c) ---- */
+/* directory holding input/ and record/ (the reference's ROOT_DIR, ".") */
+extern char bb_root_dir[FILE_NAME_SIZE];
+/* flow.config (src/domain.c:30-730) into the globals; 0 or -1 with the
+ * reference's message.  Extension: init_cond TAYLOR_GREEN (100). */
+int bb_read_flow_config(const char *path);
+/* part.config (src/particle.c:79-159): allocates and fills parts[] */
+int bb_read_part_config(const char *path);
+/* the raw restart dump of src/domain.c:3223-3413 (host arrays) */
+int bb_out_restart(const char *path);
+int bb_in_restart(const char *path);
+/* the reference's names, on ROOT_DIR/input/...; they exit on error */
+void domain_read_input(void);
+void parts_read_input(int turb);
+void out_restart(void);
+void in_restart(void);
+/* src/recorder.c:1955-2001, appending to ROOT_DIR/record/<name> */
+void recorder_lamb(char *name, int iter);
+
 #ifdef __cplusplus
 }
 #endif

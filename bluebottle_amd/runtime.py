@@ -65,6 +65,11 @@ HOST_FUNCS = {
     "bb_host_prepare": (C.c_int, []),
     "bb_host_step": (C.c_int, []),
     "compute_vel_BC": (None, []),
+    "bb_read_flow_config": (C.c_int, [C.c_char_p]),
+    "bb_read_part_config": (C.c_int, [C.c_char_p]),
+    "bb_out_restart": (C.c_int, [C.c_char_p]),
+    "bb_in_restart": (C.c_int, [C.c_char_p]),
+    "recorder_lamb": (None, [C.c_char_p, C.c_int]),
 }
 
 EXTRA_HIP = {

@@ -66,7 +66,8 @@ class Case:
     p_tDelay: float = 0.0
     g_tDelay: float = 0.0
     vel_tDelay: float = 0.0
-    init: str = "taylor_green"       # or "quiescent"
+    init: str = "taylor_green"       # or "quiescent", "shear"
+    duration: float = 1.0
     noise: float = 1e-3
     seed: int = 1
     interaction_length_ratio: float = 1.0
@@ -125,6 +126,7 @@ class Simulation:
         rs("nu", c.nu)
         rs("mu", c.nu * c.rho_f)
         rs("CFL", c.CFL)
+        rs("duration", c.duration)
         rs("pp_max_iter", int(c.pp_max_iter))
         rs("pp_residual", c.pp_residual)
         rs("lamb_max_iter", int(c.lamb_max_iter))
@@ -161,6 +163,10 @@ class Simulation:
         self.dom = runtime.dom()
         if c.init == "taylor_green":
             self._taylor_green()
+        elif c.init == "shear":
+            self._shear()
+        elif c.init != "quiescent":
+            raise ValueError("initial condition %r is not set up here" % c.init)
         n = len(c.particles)
         self.host.bb_host_particles(n)
         if n:
@@ -204,6 +210,40 @@ class Simulation:
             f = f.reshape(-1)
             seed = c.seed * 3 + {"u": 0, "v": 1, "w": 2}[comp]
             f = f + c.noise * normal_noise(seed, f.size)
+            self.array(comp)[:] = f
+            self.array(comp + "0")[:] = f
+
+    def _shear(self):
+        """Linear profiles between the Dirichlet wall velocities
+        (src/domain.c:2205-2241, init_cond SHEAR; as there, v's z term
+        divides by ze rather than the length)."""
+        c, d = self.case, self.dom
+        val = lambda spec, f: spec[f][1] if spec[f][0] == abi.DIRICHLET else 0.0
+        W, E, S, N, B, T = range(6)
+        for comp, G in (("u", d.Gfx), ("v", d.Gfy), ("w", d.Gfz)):
+            i = np.arange(G.inb)
+            j = np.arange(G.jnb)
+            k = np.arange(G.knb)
+            x = ((i - 0.5) * d.dx + d.xs)[None, None, :]
+            y = ((j - 0.5) * d.dy + d.ys)[None, :, None]
+            z = ((k - 0.5) * d.dz + d.zs)[:, None, None]
+            shape = (len(k), len(j), len(i))
+            if comp == "u":
+                sp = c.bc_u
+                f = ((val(sp, N) - val(sp, S)) * (y - d.ys) / d.yl + val(sp, S)
+                     + (val(sp, T) - val(sp, B)) * (z - d.zs) / d.zl
+                     + val(sp, B))
+            elif comp == "v":
+                sp = c.bc_v
+                f = ((val(sp, E) - val(sp, W)) * (x - d.xs) / d.xl + val(sp, W)
+                     + (val(sp, T) - val(sp, B)) * (z - d.zs) / d.ze
+                     + val(sp, B))
+            else:
+                sp = c.bc_w
+                f = ((val(sp, E) - val(sp, W)) * (x - d.xs) / d.xl + val(sp, W)
+                     + (val(sp, N) - val(sp, S)) * (y - d.ys) / d.yl
+                     + val(sp, S))
+            f = np.broadcast_to(f, shape).reshape(-1)
             self.array(comp)[:] = f
             self.array(comp + "0")[:] = f
 
