@@ -188,9 +188,13 @@ def config5(nx=384, ny=192, nz=384, nparts=8000, lamb_max_iter=25,
                 out_plane=abi.NORTH, init="shear", seed=5, particles=parts)
 
 
-def channel_case(n=(16, 12, 16), nparts=0, seed=5):
+def channel_case(n=(16, 12, 16), nparts=0, seed=5, init="taylor_green"):
     """Wall-bounded shear box: no-slip y walls (Dirichlet u = +-U_w), p
-    Neumann, periodic x and z."""
+    Neumann, periodic x and z.  Spheres of radius 2 dx are placed at least
+    one radius from either wall: nearer, or started in a Taylor-Green or
+    linear shear field at this resolution, the coupled step diverges
+    within a few steps (dt falls below 1e-10), in the oracle as on the
+    device, so multi-step runs of this case start quiescent."""
     nx, ny, nz = n
     uw = 0.5
     D, N, P = abi.DIRICHLET, abi.NEUMANN, abi.PERIODIC
@@ -202,13 +206,13 @@ def channel_case(n=(16, 12, 16), nparts=0, seed=5):
     if nparts:
         a = 2.0 / nx
         pos = place_spheres(nparts, a, (0, 0, 0), (1, 1, 1), 1.0 / nx, seed,
-                            periodic=(1, 0, 1), wall_gap=1.0 / nx)
+                            periodic=(1, 0, 1), wall_gap=a)
         parts = [Particle(x=float(p[0]), y=float(p[1]), z=float(p[2]), r=a,
                           rho=1.5) for p in pos]
     return Case(n=(nx, ny, nz), nu=0.01, CFL=0.3, pp_max_iter=2000,
                 pp_residual=1e-11, lamb_max_iter=2, bc_p=bc_p, bc_u=bc_u,
                 bc_v=bc_v, bc_w=bc_w, out_plane=abi.NORTH,
-                init="taylor_green", noise=1e-3, seed=seed, particles=parts)
+                init=init, noise=1e-3, seed=seed, particles=parts)
 
 
 # ---------------------------------------------------------------------------

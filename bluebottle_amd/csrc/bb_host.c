@@ -330,21 +330,43 @@ void compute_vel_BC(void)
   bc.wBD = ramp(delta, bc.wBDa, bc.wBDm); bc.wTD = ramp(delta, bc.wTDa, bc.wTDm);
 }
 
-/* src/bluebottle.c:445-545 after the pushes: cages, first dt, BCs. */
-int bb_host_prepare(void)
+static void first_dt_and_bcs(void)
 {
-  cuda_build_cages();
-  cuda_part_pull();
   dt = cuda_find_dt();
   if(nparts > 0) cuda_part_BC();
   cuda_dom_BC();
   cuda_parts_internal();
   cuda_dom_BC();
+}
+
+/* src/bluebottle.c:445-545 after the pushes: cages, first dt, BCs. */
+int bb_host_prepare(void)
+{
+  cuda_build_cages();
+  cuda_part_pull();
+  first_dt_and_bcs();
+  return 0;
+}
+
+/* src/bluebottle.c:449-545 with restart requested: the cages of the
+ * configured particles, then the restart file replaces the host state and
+ * is pushed, and dt is recomputed from the restored field before the
+ * boundary conditions are applied. */
+int bb_host_restart(const char *path)
+{
+  cuda_build_cages();
+  cuda_part_pull();
+  if(bb_in_restart(path) != 0) return -1;
+  cuda_dom_push();
+  cuda_part_push();
+  first_dt_and_bcs();
   return 0;
 }
 
 /* src/bluebottle.c:598-711: one time step; returns the number of Lamb
- * sub-iterations taken. */
+ * sub-iterations taken, or -1 when the new dt fails the driver's blow-up
+ * test (dt < 1e-10 or dt > 1e10, or not a number), after which the state
+ * must not be stepped further. */
 int bb_host_step(void)
 {
   ttime += dt;
@@ -390,5 +412,11 @@ int bb_host_step(void)
   if(nparts > 0) cuda_store_coeffs();
   dt0 = dt;
   dt = cuda_find_dt();
+  /* the driver's blow-up test (src/bluebottle.c:800-804) */
+  if(!(dt >= 1.e-10 && dt <= 1.e10)) {
+    printf("The solution has diverged.  Ending simulation.\n");
+    fflush(stdout);
+    return -1;
+  }
   return iter;
 }

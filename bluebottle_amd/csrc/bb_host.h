@@ -35,7 +35,9 @@ void bb_host_free(void);
 void compute_vel_BC(void);
 /* src/bluebottle.c:445-545 after the device pushes. */
 int bb_host_prepare(void);
-/* src/bluebottle.c:598-711; returns the Lamb sub-iteration count. */
+int bb_host_restart(const char *path);
+/* src/bluebottle.c:598-711; returns the Lamb sub-iteration count, or -1
+ * when the solution has diverged (the driver's dt test). */
 int bb_host_step(void);
 
 /* ---- files in the reference's formats (bb_io.c) ---- */
@@ -56,6 +58,19 @@ void out_restart(void);
 void in_restart(void);
 /* src/recorder.c:1955-2001, appending to ROOT_DIR/record/<name> */
 void recorder_lamb(char *name, int iter);
+
+/* ---- ParaView output (bb_vtk.c, src/vtk.c) into directory `dir` ---- */
+int bb_vtk_init(const char *dir);             /* empty out.pvd */
+int bb_vtk_grid(const char *dir, int step);   /* out_N.pvtr, out_N_0.vtr */
+int bb_vtk_parts(const char *dir, int step);  /* out_N.vtp */
+int bb_vtk_nodes(const char *dir, int step);  /* out_nodes_N.vtp */
+int bb_vtk_out(const char *dir, int step);    /* all of them + out.pvd */
+/* the reference's names, on ROOT_DIR/output; they exit on error */
+void init_VTK(void);
+void out_VTK(void);
+void dom_out_VTK(void);
+void part_out_VTK(void);
+void quadnodes_out_VTK(void);
 
 #ifdef __cplusplus
 }

@@ -109,6 +109,20 @@ __global__ void k_cages(BBGeom G, int n, part_struct *__restrict__ parts)
   if(i >= n) return;
   part_struct *P = &parts[i];
   cage_struct c;
+  /* a diverged particle (a centre not a number, or more than a domain
+   * length outside on some axis) gets an empty cage, so that no grid
+   * index below converts out of range; the driver's dt test ends such a
+   * run (the reference's conversions are undefined there) */
+  if(!(fabs(P->x - 0.5 * (G.xs + G.xe)) < 1.5 * G.xl
+       && fabs(P->y - 0.5 * (G.ys + G.ye)) < 1.5 * G.yl
+       && fabs(P->z - 0.5 * (G.zs + G.ze)) < 1.5 * G.zl)) {
+    c.cx = c.cy = c.cz = 0;
+    c.is = c.ibs = c.ie = c.ibe = c.in = 0;
+    c.js = c.jbs = c.je = c.jbe = c.jn = 0;
+    c.ks = c.kbs = c.ke = c.kbe = c.kn = 0;
+    P->cage = c;
+    return;
+  }
   c.in = (int)(2.0 * ceil(P->r / G.dx)) + 2;
   c.jn = (int)(2.0 * ceil(P->r / G.dy)) + 2;
   c.kn = (int)(2.0 * ceil(P->r / G.dz)) + 2;
@@ -152,9 +166,11 @@ k_build_phase(BBGeom G, int xPer, int yPer, int zPer,
   if(wx && !(c.ibe < c.ie)) return;
   if(wy && !(c.jbe < c.je)) return;
   if(wz && !(c.kbe < c.ke)) return;
-  int i0 = wx ? c.ibe : c.is, i1 = wx ? c.ie : c.ibs;
-  int j0 = wy ? c.jbe : c.js, j1 = wy ? c.je : c.jbs;
-  int k0 = wz ? c.kbe : c.ks, k1 = wz ? c.ke : c.kbs;
+  /* every valid cage lies in [1, n+1) on each axis; the clamp to the
+   * ghosted grid only guards the writes against a corrupt cage */
+  const int i0 = max(wx ? c.ibe : c.is, 0), i1 = min(wx ? c.ie : c.ibs, G.xn + 2);
+  const int j0 = max(wy ? c.jbe : c.js, 0), j1 = min(wy ? c.je : c.jbs, G.yn + 2);
+  const int k0 = max(wz ? c.kbe : c.ks, 0), k1 = min(wz ? c.ke : c.kbs, G.zn + 2);
   double X = wx ? P->x - G.xl*(double)(P->x > (G.xe - P->r))*xPer
                 : P->x + (double)(P->x < G.xs + P->r)*G.xl*xPer;
   double Y = wy ? P->y - G.yl*(double)(P->y > (G.ye - P->r))*yPer
@@ -832,14 +848,16 @@ __global__ void k_bin_fill(int n, part_struct *parts, dom_struct bd,
 {
   int pp = blockIdx.x * blockDim.x + threadIdx.x;
   if(pp >= n) return;
-  int ib = floor((parts[pp].x - bd.xs)/bd.dx);
-  int jb = floor((parts[pp].y - bd.ys)/bd.dy);
-  int kb = floor((parts[pp].z - bd.zs)/bd.dz);
   /* a particle exactly on the upper domain face maps to bin n, one past
-   * the grid (an out-of-bounds index in the reference); keep it in bin n-1 */
-  ib = min(max(ib, 0), bd.xn - 1);
-  jb = min(max(jb, 0), bd.yn - 1);
-  kb = min(max(kb, 0), bd.zn - 1);
+   * the grid (an out-of-bounds index in the reference); keep it in bin
+   * n-1.  The clamp is taken before the conversion to int, so that a
+   * position far outside (or not a number) still lands in range. */
+  const int ib = (int) fmin(fmax(floor((parts[pp].x - bd.xs)/bd.dx), 0.),
+    (double) (bd.xn - 1));
+  const int jb = (int) fmin(fmax(floor((parts[pp].y - bd.ys)/bd.dy), 0.),
+    (double) (bd.yn - 1));
+  const int kb = (int) fmin(fmax(floor((parts[pp].z - bd.zs)/bd.dz), 0.),
+    (double) (bd.zn - 1));
   int c = ib + jb*bd.Gcc.s1 + kb*bd.Gcc.s2;
   pbin[pp] = c;
   parts[pp].bin = c;

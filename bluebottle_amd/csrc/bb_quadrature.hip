@@ -85,9 +85,12 @@ struct InterpArgs {
   double rho_f, nu;
 };
 
-__device__ inline int clampi(int v, int lo, int hi)
+/* cell index from a (floored or rounded) cell coordinate, clamped to
+ * [lo, hi] before the conversion, so that no position -- however far
+ * outside the domain, or not a number -- converts out of range */
+__device__ inline int cell_clamp(double q, int lo, int hi)
 {
-  return v < lo ? lo : (v > hi ? hi : v);
+  return (int) fmin(fmax(q, (double) lo), (double) hi);
 }
 
 /* node values of one (particle, node): src/quadrature_kernel.cu:95-303 */
@@ -113,9 +116,9 @@ __device__ void interp_node(const InterpArgs &A, const part_struct *P,
   else if(z > G.ze && bc.wT == PERIODIC) z = z - G.zl;
   const double ddx = 1. / G.dx, ddy = 1. / G.dy, ddz = 1. / G.dz;
 
-  int i = clampi((int) floor((x - G.xs) * ddx) + DOM_BUF, 1, G.xn);
-  int j = clampi((int) floor((y - G.ys) * ddy) + DOM_BUF, 1, G.yn);
-  int k = clampi((int) floor((z - G.zs) * ddz) + DOM_BUF, 1, G.zn);
+  int i = cell_clamp(floor((x - G.xs) * ddx) + DOM_BUF, 1, G.xn);
+  int j = cell_clamp(floor((y - G.ys) * ddy) + DOM_BUF, 1, G.yn);
+  int k = cell_clamp(floor((z - G.zs) * ddz) + DOM_BUF, 1, G.zn);
   int C = i + j*G.cs1 + k*G.cs2;
   double xx = (i-0.5) * G.dx + G.xs;
   double yy = (j-0.5) * G.dy + G.ys;
@@ -139,9 +142,9 @@ __device__ void interp_node(const InterpArgs &A, const part_struct *P,
   double sfac = 0.1/A.nu *(rs5-a5)/rs3;
 
   /* u */
-  i = clampi((int) round((x - G.xs) * ddx) + DOM_BUF, 1, G.xn + 1);
-  j = clampi((int) floor((y - G.ys) * ddy) + DOM_BUF, 1, G.yn);
-  k = clampi((int) floor((z - G.zs) * ddz) + DOM_BUF, 1, G.zn);
+  i = cell_clamp(round((x - G.xs) * ddx) + DOM_BUF, 1, G.xn + 1);
+  j = cell_clamp(floor((y - G.ys) * ddy) + DOM_BUF, 1, G.yn);
+  k = cell_clamp(floor((z - G.zs) * ddz) + DOM_BUF, 1, G.zn);
   xx = (i-DOM_BUF) * G.dx + G.xs; yy = (j-0.5) * G.dy + G.ys;
   zz = (k-0.5) * G.dz + G.zs;
   C = i + j*G.us1 + k*G.us2;
@@ -156,9 +159,9 @@ __device__ void interp_node(const InterpArgs &A, const part_struct *P,
   uu = (nd == -1)*uu + (nd < -1)*uw;
 
   /* v */
-  i = clampi((int) floor((x - G.xs) * ddx) + DOM_BUF, 1, G.xn);
-  j = clampi((int) round((y - G.ys) * ddy) + DOM_BUF, 1, G.yn + 1);
-  k = clampi((int) floor((z - G.zs) * ddz) + DOM_BUF, 1, G.zn);
+  i = cell_clamp(floor((x - G.xs) * ddx) + DOM_BUF, 1, G.xn);
+  j = cell_clamp(round((y - G.ys) * ddy) + DOM_BUF, 1, G.yn + 1);
+  k = cell_clamp(floor((z - G.zs) * ddz) + DOM_BUF, 1, G.zn);
   xx = (i-0.5) * G.dx + G.xs; yy = (j-DOM_BUF) * G.dy + G.ys;
   zz = (k-0.5) * G.dz + G.zs;
   C = i + j*G.vs1 + k*G.vs2;
@@ -173,9 +176,9 @@ __device__ void interp_node(const InterpArgs &A, const part_struct *P,
   vv = (nd == -1)*vv + (nd < -1)*vw;
 
   /* w */
-  i = clampi((int) floor((x - G.xs) * ddx) + DOM_BUF, 1, G.xn);
-  j = clampi((int) floor((y - G.ys) * ddy) + DOM_BUF, 1, G.yn);
-  k = clampi((int) round((z - G.zs) * ddz) + DOM_BUF, 1, G.zn + 1);
+  i = cell_clamp(floor((x - G.xs) * ddx) + DOM_BUF, 1, G.xn);
+  j = cell_clamp(floor((y - G.ys) * ddy) + DOM_BUF, 1, G.yn);
+  k = cell_clamp(round((z - G.zs) * ddz) + DOM_BUF, 1, G.zn + 1);
   xx = (i-0.5) * G.dx + G.xs; yy = (j-0.5) * G.dy + G.ys;
   zz = (k-DOM_BUF) * G.dz + G.zs;
   C = i + j*G.ws1 + k*G.ws2;

@@ -63,6 +63,7 @@ HOST_FUNCS = {
     "bb_host_particles_init": (C.c_int, []),
     "bb_host_free": (None, []),
     "bb_host_prepare": (C.c_int, []),
+    "bb_host_restart": (C.c_int, [C.c_char_p]),
     "bb_host_step": (C.c_int, []),
     "compute_vel_BC": (None, []),
     "bb_read_flow_config": (C.c_int, [C.c_char_p]),
@@ -70,6 +71,11 @@ HOST_FUNCS = {
     "bb_out_restart": (C.c_int, [C.c_char_p]),
     "bb_in_restart": (C.c_int, [C.c_char_p]),
     "recorder_lamb": (None, [C.c_char_p, C.c_int]),
+    "bb_vtk_init": (C.c_int, [C.c_char_p]),
+    "bb_vtk_grid": (C.c_int, [C.c_char_p, C.c_int]),
+    "bb_vtk_parts": (C.c_int, [C.c_char_p, C.c_int]),
+    "bb_vtk_nodes": (C.c_int, [C.c_char_p, C.c_int]),
+    "bb_vtk_out": (C.c_int, [C.c_char_p, C.c_int]),
 }
 
 EXTRA_HIP = {

@@ -296,8 +296,36 @@ class Simulation:
     def prepare(self):
         self.host.bb_host_prepare()
 
+    def restart(self, path):
+        """Continue from a restart file instead of prepare() (the state
+        must be on the device already)."""
+        if self.host.bb_host_restart(str(path).encode()) != 0:
+            raise RuntimeError("could not read restart file %s" % path)
+
+    def write_restart(self, path):
+        """Pull the device state and write it as a restart file."""
+        self.pull()
+        if self.host.bb_out_restart(str(path).encode()) != 0:
+            raise RuntimeError("could not write restart file %s" % path)
+
+    def write_vtk(self, directory, stepnum):
+        """Pull the device state (when there is one) and write ParaView
+        output number `stepnum` into `directory` (out.pvd, out_N.pvtr,
+        out_N_0.vtr, out_N.vtp, out_nodes_N.vtp)."""
+        if self.device:
+            self.pull()
+        if self.host.bb_vtk_out(str(directory).encode(), int(stepnum)) != 0:
+            raise RuntimeError("could not write VTK output to %s" % directory)
+
     def step(self):
-        return self.host.bb_host_step()
+        """One time step; returns the Lamb sub-iteration count.  Raises
+        when the new dt fails the driver's blow-up test, as the reference
+        ends the run there."""
+        it = self.host.bb_host_step()
+        if it < 0:
+            raise RuntimeError("the solution has diverged (dt = %g)"
+                               % runtime.get_scalar("dt"))
+        return it
 
     def pull(self):
         self.hip.cuda_dom_pull()

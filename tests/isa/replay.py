@@ -58,6 +58,8 @@ class Library:
         dm = demangle(list(self.kernels))
         self.by_sig = {}
         for mangled, d in dm.items():
+            if "(" not in d:
+                continue   # a name this c++filt cannot demangle (bf16 vectors)
             base = d.split("(")[0].split("<")[0]
             if base.startswith("void "):
                 base = base[5:]   # template instances print their return type

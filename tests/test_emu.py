@@ -30,7 +30,8 @@ for name, case in (
         ("particles", configs.small_particle_case(n=12, nparts=2,
                                                   radius_dx=2.5,
                                                   lamb_max_iter=2)),
-        ("channel", configs.channel_case(n=(12, 8, 10), nparts=1))):
+        ("channel", configs.channel_case(n=(12, 8, 10), nparts=1,
+                                         init="quiescent"))):
     s = sim.Simulation(case)
     s.to_device()
     s.prepare()

@@ -28,8 +28,8 @@ def _implicit(case):
 CASES = {
     "tg_free": lambda: _implicit(configs.periodic_case(
         16, 0.05, CFL=0.3, pp_max_iter=500, pp_residual=1e-12)),
-    "channel": lambda: _implicit(configs.channel_case(n=(16, 12, 16),
-                                                      nparts=2)),
+    "channel": lambda: _implicit(configs.channel_case(
+        n=(16, 12, 16), nparts=2, init="quiescent")),
     "channel_free": lambda: _implicit(configs.channel_case(n=(16, 12, 16),
                                                            nparts=0)),
     "settling": lambda: _implicit(configs.small_particle_case(

@@ -49,7 +49,8 @@ CASES = {
         n=24, nparts=3, radius_dx=3.0, taylor_green=False, lamb_max_iter=2),
     "tg_particles": lambda: configs.small_particle_case(
         n=24, nparts=2, radius_dx=3.0, taylor_green=True, lamb_max_iter=2),
-    "channel": lambda: configs.channel_case(n=(16, 12, 16), nparts=2),
+    "channel": lambda: configs.channel_case(n=(16, 12, 16), nparts=2,
+                                            init="quiescent"),
     "channel_free": lambda: configs.channel_case(n=(16, 12, 16), nparts=0),
     "tg_free": lambda: configs.periodic_case(16, 0.01, CFL=0.3, pp_max_iter=500,
                                              pp_residual=1e-11),

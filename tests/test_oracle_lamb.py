@@ -222,3 +222,51 @@ def test_stokes_straining_flow():
         lim = 0.05 if name.startswith("pnm") else 0.01
         assert c.max() < lim * dom, (name, c.max() / dom)
     _check_part_bc(o, 0.02)
+
+
+def test_stokes_rotlet_and_the_chi_radial_factor():
+    """A fixed sphere in a rigidly rotating fluid: u = W x x (1 - a^3/r^3),
+    p = 0, whose disturbance is the n = 1 toroidal (chi) harmonic.  The fit
+    must give chi_10 alone and the torque 8 pi mu a^3 W.  The velocity that
+    part_BC rebuilds from chi carries the reference's radial factor
+    (r/a)^(n-1) - (a/r)^(n+1) (src/particle_kernel.cu:1346-1364), while the
+    fit divides by (r/a)^n - (a/r)^(n+1) (src/quadrature_kernel.cu:521-523);
+    so at the flagged faces the rebuilt velocity is the exact one times
+    (1 - (a/r)^2) / (r/a - (a/r)^2) = a (r + a) / (r^2 + r a + a^2), which
+    tends to 2/3 at the surface."""
+    W = 0.01
+    a = A_RADIUS
+
+    def field(X, Y, Z):
+        r = np.sqrt(X * X + Y * Y + Z * Z)
+        f = 1 - a ** 3 / r ** 3
+        return np.stack([-W * Y * f, W * X * f, 0 * X]), 0 * X
+
+    o = _run_fit(field)
+    P = o.parts()[0]
+    torque = 8 * math.pi * 1.0 * a ** 3 * W
+    assert abs(P.Lz / torque - 1.0) < 0.01
+    assert abs(P.Lx) < 1e-3 * torque and abs(P.Ly) < 1e-3 * torque
+    dom = abs(o.array("chinm_re")[1])
+    assert dom > 0
+    for name in COEFFS:
+        c = np.abs(o.array(name))
+        if name == "chinm_re":
+            c = np.delete(c, 1)
+        assert c.max() < 0.01 * dom, (name, c.max() / dom)
+    exact = {c: o.array(c).copy() for c in ("u", "v")}
+    o.call("part_BC")
+    s = sim.Simulation(_sphere_case())
+    d = s.dom
+    for c, G, off in (("u", d.Gfx, (1.0, 0.5, 0.5)), ("v", d.Gfy, (0.5, 1.0, 0.5))):
+        X, Y, Z = _coords(d, G, *off)
+        r = np.sqrt(X * X + Y * Y + Z * Z).ravel()
+        A, B = exact[c], o.array(c)
+        changed = A != B
+        assert changed.sum() > 100
+        factor = a * (r + a) / (r * r + r * a + a * a)
+        err = np.abs(B - A * factor)[changed].max()
+        assert err <= 0.02 * np.abs(A[changed]).max(), (c, err)
+        # and not the exact field: the factor is far from 1 near the surface
+        assert np.abs(B - A)[changed].max() > 0.2 * np.abs(A[changed]).max()
+    s.close()
