@@ -49,12 +49,18 @@ def kernel_bytes(n, xn, yn, zn):
         "u_star": 180 * n,
         # u*, v*, w*, phase, phase_shell, flag_u/v/w, phi -> rhs, x, flags
         "pp_setup": (24 + 8 + 12 + 8 + 8 + 8 + 1) * n,
+        # three sweeps per iteration (FP64, and FP32 by default):
         # r, p, v, r*, flags -> p, v  (the p update folded into the sweep)
         "spmv_p": (6 * 8 + 1) * n, "spmv_p32": (6 * 4 + 1) * n,
         # r, v, flags -> t
         "spmv_s": (3 * 8 + 1) * n, "spmv_s32": (3 * 4 + 1) * n,
         # x, p, r, v, t, r*, flags -> x, r
         "update_xr": (8 * 8 + 1) * n, "update_xr32": (8 * 4 + 1) * n,
+        # the two-sweep FP32 form (--inner-sweeps 2):
+        # sweep A: r, p, v, t, x, r*, flags -> r', x, p', v'
+        "sweep_a32": (10 * 4 + 1) * n,
+        # sweep B: r', v', r*, flags -> t
+        "sweep_b32": (4 * 4 + 1) * n,
         # b, x, flags -> r (FP64)
         "resid64": (3 * 8 + 1) * n,
         # x, e -> x
@@ -382,6 +388,9 @@ def main():
     ap.add_argument("--mixed-inner-tol", type=float, default=0.0,
                     help="relative tolerance of each FP32 inner solve of the "
                          "mixed form (default: the library's)")
+    ap.add_argument("--inner-sweeps", type=int, default=3, choices=(2, 3),
+                    help="sweeps per iteration of the FP32 inner BiCGSTAB of "
+                         "the mixed and learned forms")
     ap.add_argument("--poll", type=int, default=0,
                     help="BiCGSTAB iterations between host convergence checks "
                          "(default: the library's)")
@@ -432,6 +441,7 @@ def main():
             s.hip.bb_set_mixed_params(args.mixed_inner_tol, 0, 0)
         if args.poll > 0:
             s.hip.bb_set_poll_interval(args.poll)
+        s.hip.bb_set_inner_sweeps(args.inner_sweeps)
         s.to_device()
         s.prepare()
         s.hip.bb_synchronize()

@@ -61,7 +61,8 @@ struct KrylovScalars {
   double rho, alpha, omega, beta;
   double tol, resid, bnorm;
   double sum0, sum1, sum2;
-  int iter, max_iter, done, s_conv, converged, pad;
+  int iter, max_iter, done, s_conv, converged;
+  int cur;                  /* two-sweep form: buffer set of the last iteration */
 };
 
 struct BBWorkspace {
@@ -70,7 +71,7 @@ struct BBWorkspace {
   double *x, *r, *rs, *p, *v, *t, *b;   /* FP64 Krylov vectors */
   double *p2, *v2;          /* second buffers of p and v (ping-pong) */
   float *xf, *rf, *rsf, *pf, *vf, *tf, *bf;  /* FP32 inner-solver vectors */
-  float *pf2, *vf2;
+  float *pf2, *vf2, *rf2;   /* second buffers of the two-sweep form */
   float *phf, *shf;         /* preconditioned p and s of the flexible solve */
   uint8_t *flags;
   double *partials;
@@ -132,7 +133,7 @@ enum {
   BB_K_SPMV_P32, BB_K_SPMV_S32, BB_K_UPDATE_XR32, BB_K_UPDATE_P32,
   BB_K_PROJECT, BB_K_UPDATEP, BB_K_FINDDT, BB_K_CAGE, BB_K_PARTBC,
   BB_K_QUAD, BB_K_MOVE, BB_K_MISC, BB_K_UNET, BB_K_UNET_CONV0, BB_K_FSPMV,
-  BB_K_FVEC, BB_K_NKINDS
+  BB_K_FVEC, BB_K_SWEEP_A32, BB_K_SWEEP_B32, BB_K_NKINDS
 };
 static_assert((int) BB_K_NKINDS == (int) BB_PROF_NKINDS &&
   (int) BB_K_SPMV_S32 == (int) BB_PROF_SPMV_S32 &&

@@ -173,6 +173,7 @@ void bb_ensure_workspace(void)
   ws->vf = (float *) dmalloc(n * sizeof(float));
   ws->pf2 = (float *) dmalloc(n * sizeof(float));
   ws->vf2 = (float *) dmalloc(n * sizeof(float));
+  ws->rf2 = (float *) dmalloc(n * sizeof(float));
   ws->tf = (float *) dmalloc(n * sizeof(float));
   ws->bf = (float *) dmalloc(n * sizeof(float));
   ws->phf = (float *) dmalloc(n * sizeof(float));
@@ -191,8 +192,8 @@ void bb_free_workspace(void)
 {
   BBWorkspace *ws = &g_bb.ws;
   void *ptrs[] = {ws->x, ws->r, ws->rs, ws->p, ws->v, ws->t, ws->b, ws->xf,
-    ws->rf, ws->rsf, ws->pf, ws->vf, ws->tf, ws->bf, ws->flags, ws->partials, ws->sc,
-    ws->red, ws->p2, ws->v2, ws->pf2, ws->vf2, ws->phf, ws->shf};
+    ws->rf, ws->rsf, ws->pf, ws->vf, ws->tf, ws->bf, ws->flags, ws->partials,
+    ws->sc, ws->red, ws->p2, ws->v2, ws->pf2, ws->vf2, ws->rf2, ws->phf, ws->shf};
   for(size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) dfree(ptrs[i]);
   if(ws->sc_host) BB_CHECK(hipHostFree(ws->sc_host));
   if(ws->red_host) BB_CHECK(hipHostFree(ws->red_host));

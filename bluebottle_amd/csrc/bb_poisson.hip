@@ -304,12 +304,43 @@ __global__ void k_f_init(int nblk, const double *part, KrylovScalars *sc,
   }
 }
 
+/* The scalar steps of one iteration, from the summed partials: alpha
+ * after <r*, v>; omega and the s-convergence test after <t,s>, <t,t>,
+ * <s,s>; beta, rho and the convergence test after <r*, r>, <r, r>. */
+__device__ inline void fin_alpha(const double *o, KrylovScalars *sc)
+{
+  sc->alpha = sc->rho / o[0];
+}
+
+__device__ inline void fin_omega(const double *o, KrylovScalars *sc)
+{
+  double rs = sqrt(o[2]);
+  if(rs <= sc->tol) {   /* monitor.finished(s) */
+    sc->s_conv = 1;
+    sc->resid = rs;
+    sc->converged = 1;
+  }
+  sc->omega = o[0] / o[1];
+}
+
+__device__ inline void fin_beta(const double *o, KrylovScalars *sc)
+{
+  if(sc->s_conv) { sc->done = 1; return; }
+  double rho_new = o[0];
+  sc->beta = (rho_new / sc->rho) * (sc->alpha / sc->omega);
+  sc->rho = rho_new;
+  sc->iter += 1;
+  sc->resid = sqrt(o[1]);
+  sc->converged = sc->resid <= sc->tol;
+  sc->done = sc->converged || sc->iter >= sc->max_iter;
+}
+
 __global__ void k_f1(int nblk, const double *part, KrylovScalars *sc)
 {
   if(sc->done) return;
   double o[1];
   sum_partials<1>(part, nblk, o);
-  if(threadIdx.x == 0) sc->alpha = sc->rho / o[0];
+  if(threadIdx.x == 0) fin_alpha(o, sc);
 }
 
 __global__ void k_f2(int nblk, const double *part, KrylovScalars *sc)
@@ -317,15 +348,7 @@ __global__ void k_f2(int nblk, const double *part, KrylovScalars *sc)
   if(sc->done) return;
   double o[3];
   sum_partials<3>(part, nblk, o);
-  if(threadIdx.x == 0) {
-    double rs = sqrt(o[2]);
-    if(rs <= sc->tol) {   /* monitor.finished(s) */
-      sc->s_conv = 1;
-      sc->resid = rs;
-      sc->converged = 1;
-    }
-    sc->omega = o[0] / o[1];
-  }
+  if(threadIdx.x == 0) fin_omega(o, sc);
 }
 
 __global__ void k_f3(int nblk, const double *part, KrylovScalars *sc)
@@ -333,16 +356,7 @@ __global__ void k_f3(int nblk, const double *part, KrylovScalars *sc)
   if(sc->done) return;
   double o[2];
   sum_partials<2>(part, nblk, o);
-  if(threadIdx.x == 0) {
-    if(sc->s_conv) { sc->done = 1; return; }
-    double rho_new = o[0];
-    sc->beta = (rho_new / sc->rho) * (sc->alpha / sc->omega);
-    sc->rho = rho_new;
-    sc->iter += 1;
-    sc->resid = sqrt(o[1]);
-    sc->converged = sc->resid <= sc->tol;
-    sc->done = sc->converged || sc->iter >= sc->max_iter;
-  }
+  if(threadIdx.x == 0) fin_beta(o, sc);
 }
 
 /* ---------------------------------------------------------------------
@@ -356,13 +370,15 @@ __device__ inline double mval(const T *__restrict__ y,
   return dinv[fl[idx] & 127] * (double) y[idx];
 }
 
-/* r = b/nrm - A (x/nrm); x and b are scaled in place; r* = p = r; v = 0 */
+/* r = b/nrm - A (x/nrm); x and b are scaled in place; r* = p = r; v = 0
+ * (and t = 0 when tz is given) */
 template<typename T>
 __global__ void __launch_bounds__(256)
 k_init(PPGeom G, DinvTable tab, const KrylovScalars *sc,
   T *__restrict__ b, T *__restrict__ x, const uint8_t *__restrict__ fl,
   T *__restrict__ r, T *__restrict__ rs, T *__restrict__ p,
-  T *__restrict__ v, double *__restrict__ part, int scale_inputs)
+  T *__restrict__ v, double *__restrict__ part, int scale_inputs,
+  T *__restrict__ tz)
 {
   __shared__ double sh[4];
   __shared__ double dinv[256];
@@ -387,6 +403,7 @@ k_init(PPGeom G, DinvTable tab, const KrylovScalars *sc,
       double bc_ = (double) b[C] * inv;
       T rv = (T) (1. * bc_ + -1. * Ax);
       r[C] = rv; rs[C] = rv; p[C] = rv; v[C] = (T) 0;
+      if(tz) tz[C] = (T) 0;
       bb += bc_ * bc_;
       rr += (double) rv * (double) rv;
     }
@@ -425,7 +442,11 @@ k_scale2(size_t n, const KrylovScalars *sc, T *__restrict__ x,
  * unscaled s (K2's dots) and the cell's operator flags f.  Sweeps load the
  * raw inputs of a cell one iteration before they form its value, so that
  * no load is waited on in the iteration that issues it. */
-struct CellVal { double m, s; uint8_t f; };
+struct CellVal {
+  double m, s;        /* operand value m = M y, and the unscaled y */
+  uint8_t f;
+  double p0, sv, rn;  /* two-sweep form only: old p, s, new r */
+};
 
 template<typename T>
 struct OperandP {
@@ -440,7 +461,7 @@ struct OperandP {
   }
   __device__ inline CellVal val(const Raw &w, const double *dinv) const
   {
-    CellVal c;
+    CellVal c = {};
     c.f = w.f;
     c.m = dinv[w.f & 127] * (double) w.p;
     c.s = 0.;
@@ -468,7 +489,7 @@ struct OperandPNew {
   }
   __device__ inline CellVal val(const Raw &w, const double *dinv) const
   {
-    CellVal c;
+    CellVal c = {};
     c.f = w.f;
     c.s = (double) (T) (1. * (double) w.r + beta * (double) w.p +
       bo * (double) w.v);
@@ -494,7 +515,7 @@ struct OperandS {
   }
   __device__ inline CellVal val(const Raw &w, const double *dinv) const
   {
-    CellVal c;
+    CellVal c = {};
     c.f = w.f;
     c.s = (double) (T) (1. * (double) w.r + -alpha * (double) w.v);
     c.m = dinv[w.f & 127] * c.s;
@@ -638,7 +659,7 @@ __device__ inline void sweep_z(const PPGeom &G, const SweepTile &t,
         P[t.ty + 1][t.tx], P[t.ty + 1][t.tx + 2],
         P[t.ty][t.tx + 1], P[t.ty + 2][t.tx + 1],
         m_prev, above.m, diag_of(dinv, cur.f));
-      out(col + (long) k * s2, y, cur.s);
+      out(col + (long) k * s2, y, cur);
     }
     m_prev = cur.m; cur = above;
     raw_a = raw_nn; hraw = hraw_n;
@@ -663,7 +684,7 @@ struct OutP {
   const T *__restrict__ rs;
   T *__restrict__ v;
   double acc;
-  __device__ inline void operator()(long C, double y, double) {
+  __device__ inline void operator()(long C, double y, const CellVal &) {
     const T vt = (T) y;
     v[C] = vt;
     acc += (double) rs[C] * (double) vt;
@@ -674,8 +695,9 @@ template<typename T>
 struct OutS {
   T *__restrict__ t;
   double ts, tt, ss;
-  __device__ inline void operator()(long C, double y, double s) {
+  __device__ inline void operator()(long C, double y, const CellVal &c) {
     const T tv = (T) y;
+    const double s = c.s;
     t[C] = tv;
     ts += (double) tv * s;
     tt += (double) tv * (double) tv;
@@ -689,9 +711,9 @@ struct OutPNew {
   T *__restrict__ p;
   T *__restrict__ v;
   double acc;
-  __device__ inline void operator()(long C, double y, double pn) {
+  __device__ inline void operator()(long C, double y, const CellVal &c) {
     const T vt = (T) y;
-    p[C] = (T) pn;
+    p[C] = (T) c.s;
     v[C] = vt;
     acc += (double) rs[C] * (double) vt;
   }
@@ -793,6 +815,220 @@ k_update_xr(PPGeom G, DinvTable tab, const KrylovScalars *sc,
 }
 
 /* ---------------------------------------------------------------------
+ * Two-sweep BiCGSTAB (the FP32 inner solve of the mixed form).
+ *
+ * The update sweep K3 exists only to produce r' = s - omega t and its dots
+ * <r*, r'> and <r', r'>, which beta and the convergence test need before
+ * the next K1.  Both follow from dots the K2 sweep can take, by linearity:
+ *   <r*, r'> = <r*, s> - omega <r*, t>
+ *   <r', r'> = <s, s> - 2 omega <t, s> + omega^2 <t, t>
+ * (each evaluated afresh from this iteration's vectors, not a recurrence).
+ * So K2 takes five dots and the finishing block computes omega, beta and
+ * the residual at once; the next iteration's first sweep forms
+ * s = r - alpha v, r' = s - omega t and x += alpha M p + omega M s on the
+ * fly for its own cells, then p' = r' + beta (p - omega v) and v' = A M p'
+ * as K1 does.  Every vector is rounded to T exactly where the three-sweep
+ * form rounds it; r, p and v alternate between two buffer sets, because a
+ * block reads its halo cells' old values while their owners write new
+ * ones.  Per iteration: two sweeps and 58 B/cell at FP32 (three sweeps and
+ * 71 B/cell before).  After the last iteration one pass applies the
+ * pending x update.
+ * ------------------------------------------------------------------- */
+template<typename T>
+struct OperandA {
+  struct Raw { T r, p, v, t; uint8_t f; };
+  const T *__restrict__ r;
+  const T *__restrict__ p;
+  const T *__restrict__ v;
+  const T *__restrict__ t;
+  double alpha, omega, beta, bo;
+  __device__ inline Raw load(const uint8_t *__restrict__ fl, long C) const
+  {
+    Raw w;
+    w.r = r[C];
+    w.p = p[C];
+    w.v = v[C];
+    w.t = t[C];
+    w.f = fl[C];
+    return w;
+  }
+  __device__ inline CellVal val(const Raw &w, const double *dinv) const
+  {
+    CellVal c = {};
+    c.f = w.f;
+    const T sv = (T) (1. * (double) w.r + -alpha * (double) w.v);
+    const T rn = (T) (1. * (double) sv + -omega * (double) w.t);
+    c.s = (double) (T) (1. * (double) rn + beta * (double) w.p +
+      bo * (double) w.v);
+    c.m = dinv[w.f & 127] * c.s;
+    c.p0 = (double) w.p;
+    c.sv = (double) sv;
+    c.rn = (double) rn;
+    return c;
+  }
+};
+
+template<typename T>
+struct OutA {
+  const T *__restrict__ rs;
+  T *__restrict__ x;
+  T *__restrict__ r_new;
+  T *__restrict__ p_new;
+  T *__restrict__ v_new;
+  const double *dinv;
+  double alpha, omega;
+  double acc;
+  __device__ inline void operator()(long C, double y, const CellVal &c) {
+    const double di = dinv[c.f & 127];
+    x[C] = (T) (1. * (double) x[C] + alpha * (di * c.p0) + omega * (di * c.sv));
+    r_new[C] = (T) c.rn;
+    p_new[C] = (T) c.s;
+    const T vt = (T) y;
+    v_new[C] = vt;
+    acc += (double) rs[C] * (double) vt;
+  }
+};
+
+template<typename T>
+struct OutB {
+  const T *__restrict__ rs;
+  T *__restrict__ t;
+  double ts, tt, ss, rss, rst;
+  __device__ inline void operator()(long C, double y, const CellVal &c) {
+    const T tv = (T) y;
+    const double s = c.s, rv = (double) rs[C];
+    t[C] = tv;
+    ts += (double) tv * s;
+    tt += (double) tv * (double) tv;
+    ss += s * s;
+    rss += rv * s;
+    rst += rv * (double) tv;
+  }
+};
+
+/* omega, the s-convergence test, then beta and the convergence test from
+ * the five dots <t,s>, <t,t>, <s,s>, <r*,s>, <r*,t>; `set` is the buffer
+ * set this iteration's vectors are in */
+__device__ inline void fin_two_sweep(const double *o, KrylovScalars *sc,
+  int set)
+{
+  sc->cur = set;
+  const double rs_ = sqrt(o[2]);
+  if(rs_ <= sc->tol) {   /* monitor.finished(s) */
+    sc->s_conv = 1;
+    sc->resid = rs_;
+    sc->converged = 1;
+    sc->done = 1;
+    return;
+  }
+  const double omega = o[0] / o[1];
+  sc->omega = omega;
+  const double rho_new = o[3] - omega * o[4];
+  const double rr = o[2] - 2. * omega * o[0] + omega * omega * o[1];
+  sc->beta = (rho_new / sc->rho) * (sc->alpha / omega);
+  sc->rho = rho_new;
+  sc->iter += 1;
+  sc->resid = sqrt(rr > 0. ? rr : 0.);
+  sc->converged = sc->resid <= sc->tol;
+  sc->done = sc->converged || sc->iter >= sc->max_iter;
+}
+
+/* sweep A: pending update of the previous iteration, then K1 */
+template<typename T>
+__global__ void __launch_bounds__(SX * SY)
+k_sweep_a(PPGeom G, DinvTable tab, const KrylovScalars *sc,
+  const T *__restrict__ r, const T *__restrict__ p, const T *__restrict__ v,
+  const T *__restrict__ t, const T *__restrict__ rs,
+  const uint8_t *__restrict__ fl, T *__restrict__ x, T *__restrict__ r_new,
+  T *__restrict__ p_new, T *__restrict__ v_new, double *__restrict__ part)
+{
+  if(sc->done) return;
+  __shared__ double sh[SX * SY / 64];
+  __shared__ double dinv[256];
+  __shared__ double pl[2][SY + 2][SX + 2];
+  load_dinv(tab, dinv);
+  const int lb = xcd_block(G.snblk);
+  const SweepTile tt = sweep_tile(G, lb);
+  const double alpha = sc->alpha, omega = sc->omega, beta = sc->beta;
+  OperandA<T> op{r, p, v, t, alpha, omega, beta, -beta * omega};
+  OutA<T> out{rs, x, r_new, p_new, v_new, dinv, alpha, omega, 0.};
+  sweep_z(G, tt, fl, dinv, pl, op, out);
+  const double a = sweep_block_sum(out.acc, sh);
+  if(threadIdx.x == 0 && threadIdx.y == 0) part[lb] = a;
+}
+
+/* sweep B: s = r - alpha v, t = A M s and the five dots */
+template<typename T>
+__global__ void __launch_bounds__(SX * SY)
+k_sweep_b(PPGeom G, DinvTable tab, const KrylovScalars *sc,
+  const T *__restrict__ r, const T *__restrict__ v,
+  const T *__restrict__ rs, const uint8_t *__restrict__ fl,
+  T *__restrict__ t_out, double *__restrict__ part)
+{
+  if(sc->done) return;
+  __shared__ double sh[SX * SY / 64];
+  __shared__ double dinv[256];
+  __shared__ double pl[2][SY + 2][SX + 2];
+  load_dinv(tab, dinv);
+  const int lb = xcd_block(G.snblk);
+  const SweepTile t = sweep_tile(G, lb);
+  OperandS<T> op{r, v, sc->alpha};
+  OutB<T> out{rs, t_out, 0., 0., 0., 0., 0.};
+  sweep_z(G, t, fl, dinv, pl, op, out);
+  const double q0 = sweep_block_sum(out.ts, sh);
+  const double q1 = sweep_block_sum(out.tt, sh);
+  const double q2 = sweep_block_sum(out.ss, sh);
+  const double q3 = sweep_block_sum(out.rss, sh);
+  const double q4 = sweep_block_sum(out.rst, sh);
+  if(threadIdx.x == 0 && threadIdx.y == 0) {
+    part[5*lb] = q0; part[5*lb+1] = q1; part[5*lb+2] = q2;
+    part[5*lb+3] = q3; part[5*lb+4] = q4;
+  }
+}
+
+/* the finishing kernel of sweep B */
+__global__ void k_f_two(int nblk, const double *part, KrylovScalars *sc,
+  int set)
+{
+  if(sc->done) return;
+  double o[5];
+  sum_partials<5>(part, nblk, o);
+  if(threadIdx.x == 0) fin_two_sweep(o, sc, set);
+}
+
+/* the update the last iteration left pending: x += alpha M p (+ omega M s
+ * with s = r - alpha v, unless s converged), on buffer set sc->cur */
+template<typename T>
+__global__ void __launch_bounds__(256)
+k_final_x(size_t n, DinvTable tab, const KrylovScalars *sc,
+  const uint8_t *__restrict__ fl, T *__restrict__ x,
+  const T *r0, const T *p0, const T *v0, const T *r1, const T *p1,
+  const T *v1)
+{
+  __shared__ double dinv[256];
+  if(threadIdx.x < 256)
+    dinv[threadIdx.x] = threadIdx.x < 128 ? tab.d[threadIdx.x]
+                                          : tab.g[threadIdx.x - 128];
+  __syncthreads();
+  if(sc->iter == 0 && !sc->s_conv) return;   /* no iteration ran */
+  const int set = sc->cur;
+  const T *r = set ? r1 : r0, *p = set ? p1 : p0, *v = set ? v1 : v0;
+  const double alpha = sc->alpha, omega = sc->omega;
+  const int sconv = sc->s_conv;
+  size_t stride = (size_t) gridDim.x * blockDim.x;
+  for(size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+    const double di = dinv[fl[q] & 127];
+    const double Mp = di * (double) p[q];
+    if(sconv) {
+      x[q] = (T) (1. * (double) x[q] + alpha * Mp);
+      continue;
+    }
+    const T sv = (T) (1. * (double) r[q] + -alpha * (double) v[q]);
+    x[q] = (T) (1. * (double) x[q] + alpha * Mp + omega * (di * (double) sv));
+  }
+}
+
+/* ---------------------------------------------------------------------
  * Flexible BiCGSTAB (right preconditioner M that may change between
  * applications: the learned network).  Per iteration
  *   p^ = M p;  v = A p^, <r*,v>;  alpha
@@ -816,7 +1052,7 @@ struct OperandPlain {
   }
   __device__ inline CellVal val(const Raw &w, const double *) const
   {
-    CellVal c;
+    CellVal c = {};
     c.f = w.f;
     c.m = (double) w.p;
     c.s = 0.;
@@ -829,7 +1065,7 @@ struct OutTS {
   T *__restrict__ t;
   const T *__restrict__ s;
   double ts, tt;
-  __device__ inline void operator()(long C, double y, double) {
+  __device__ inline void operator()(long C, double y, const CellVal &) {
     const T tv = (T) y;
     t[C] = tv;
     const double sv = (double) s[C];
@@ -1114,7 +1350,7 @@ static void bicgstab_run(const PPGeom &G, const DinvTable &tab, T *b, T *x,
     exit(EXIT_FAILURE);
   }
   k_init<T><<<G.nblk, blk, 0, st>>>(G, tab, ws->sc, b, x, ws->flags, r, rs,
-    p, v, ws->partials, scale_inputs);
+    p, v, ws->partials, scale_inputs, (T *) nullptr);
   BB_CHECK_LAUNCH();
   if(scale_inputs) {
     k_scale2<T><<<eb, 256, 0, st>>>(n, ws->sc, x, b);
@@ -1161,7 +1397,90 @@ static void bicgstab_run(const PPGeom &G, const DinvTable &tab, T *b, T *x,
   BB_CHECK(hipStreamSynchronize(st));
 }
 
+/* Two-sweep BiCGSTAB of A x = b on vectors of type T (b and x already
+ * normalised, x holding the initial guess); r, p and v alternate between
+ * the sets (r0, p0, v0) and (r1, p1, v1).  Leaves the final scalars in
+ * ws->sc_host. */
+template<typename T>
+static void bicgstab2_run(const PPGeom &G, const DinvTable &tab, T *b, T *x,
+  T *r0, T *r1, T *rs, T *p0, T *p1, T *v0, T *v1, T *t, double rel_tol,
+  int max_iter, int poll, const int kinds[2])
+{
+  BBWorkspace *ws = &g_bb.ws;
+  hipStream_t st = g_bb.stream;
+  dim3 blk(PX, PY);
+  dim3 sblk(SX, SY);
+  size_t n = ws->n;
+  if(G.snblk > ws->nblk || 5 * G.snblk > 4 * ws->nblk) {
+    fprintf(stderr, "bluebottle_hip: %d sweep tiles exceed the %d reduction "
+      "slots\n", G.snblk, ws->nblk);
+    exit(EXIT_FAILURE);
+  }
+  k_init<T><<<G.nblk, blk, 0, st>>>(G, tab, ws->sc, b, x, ws->flags, r0, rs,
+    p0, v0, ws->partials, 0, t);
+  BB_CHECK_LAUNCH();
+  k_f_init<<<1, FIN_THREADS, 0, st>>>(G.nblk, ws->partials, ws->sc, rel_tol,
+    max_iter);
+  BB_CHECK_LAUNCH();
+  T *r[2] = {r0, r1}, *p[2] = {p0, p1}, *v[2] = {v0, v1};
+  for(int it = 0; it < max_iter + 1; it++) {
+    const int a = it & 1, c = a ^ 1;
+    bb_prof_begin(kinds[0]);
+    k_sweep_a<T><<<G.snblk, sblk, 0, st>>>(G, tab, ws->sc, r[a], p[a], v[a],
+      t, rs, ws->flags, x, r[c], p[c], v[c], ws->partials);
+    BB_CHECK_LAUNCH();
+    bb_prof_end(kinds[0]);
+    k_f1<<<1, FIN_THREADS, 0, st>>>(G.snblk, ws->partials, ws->sc);
+    BB_CHECK_LAUNCH();
+    bb_prof_begin(kinds[1]);
+    k_sweep_b<T><<<G.snblk, sblk, 0, st>>>(G, tab, ws->sc, r[c], v[c], rs,
+      ws->flags, t, ws->partials);
+    BB_CHECK_LAUNCH();
+    bb_prof_end(kinds[1]);
+    k_f_two<<<1, FIN_THREADS, 0, st>>>(G.snblk, ws->partials, ws->sc, c);
+    BB_CHECK_LAUNCH();
+    if((it + 1) % poll == 0 || it == max_iter) {
+      BB_CHECK(hipMemcpyAsync(ws->sc_host, ws->sc, sizeof(KrylovScalars),
+        hipMemcpyDeviceToHost, st));
+      BB_CHECK(hipStreamSynchronize(st));
+      if(ws->sc_host->done) break;
+    }
+  }
+  k_final_x<T><<<elementwise_blocks(n), 256, 0, st>>>(n, tab, ws->sc,
+    ws->flags, x, r0, p0, v0, r1, p1, v1);
+  BB_CHECK_LAUNCH();
+  BB_CHECK(hipMemcpyAsync(ws->sc_host, ws->sc, sizeof(KrylovScalars),
+    hipMemcpyDeviceToHost, st));
+  BB_CHECK(hipStreamSynchronize(st));
+}
+
 static int g_poll = 8;
+static int g_inner_sweeps = 3;
+
+extern "C" int bb_set_inner_sweeps(int n)
+{
+  const int old = g_inner_sweeps;
+  if(n == 2 || n == 3) g_inner_sweeps = n;
+  return old;
+}
+
+/* The FP32 Jacobi BiCGSTAB of the mixed form's inner solves (and of the
+ * learned form's fallback): A x = b from x, b and x normalised. */
+static void inner_jacobi(const PPGeom &G, const DinvTable &tab, float *b,
+  float *x, double rel_tol, int max_iter)
+{
+  BBWorkspace *ws = &g_bb.ws;
+  if(g_inner_sweeps == 2) {
+    const int kinds[2] = {BB_K_SWEEP_A32, BB_K_SWEEP_B32};
+    bicgstab2_run<float>(G, tab, b, x, ws->rf, ws->rf2, ws->rsf, ws->pf,
+      ws->pf2, ws->vf, ws->vf2, ws->tf, rel_tol, max_iter, g_poll, kinds);
+  } else {
+    int kinds[4] = {BB_K_SPMV_P32, BB_K_SPMV_S32, BB_K_UPDATE_XR32,
+      BB_K_UPDATE_P32};
+    bicgstab_run<float>(G, tab, b, x, ws->rf, ws->rsf, ws->pf, ws->vf,
+      ws->pf2, ws->vf2, ws->tf, rel_tol, max_iter, 0, g_poll, kinds);
+  }
+}
 
 extern "C" void bb_set_poll_interval(int n) { g_poll = n > 0 ? n : 1; }
 
@@ -1264,7 +1583,7 @@ static void flexible_run(const PPGeom &G, const DinvTable &tab, double rel_tol,
     exit(EXIT_FAILURE);
   }
   k_init<float><<<G.nblk, blk, 0, st>>>(G, tab, ws->sc, b, x, ws->flags, r,
-    rs, p, v, ws->partials, 0);
+    rs, p, v, ws->partials, 0, (float *) nullptr);
   BB_CHECK_LAUNCH();
   k_f_init<<<1, FIN_THREADS, 0, st>>>(G.nblk, ws->partials, ws->sc, rel_tol,
     max_iter);
@@ -1323,10 +1642,7 @@ static void flexible_run(const PPGeom &G, const DinvTable &tab, double rel_tol,
         int done_it = ws->sc_host->iter;
         g_learn.flex_iters += done_it;
         g_learn.fallbacks++;
-        int kinds[4] = {BB_K_SPMV_P32, BB_K_SPMV_S32, BB_K_UPDATE_XR32,
-          BB_K_UPDATE_P32};
-        bicgstab_run<float>(G, tab, b, x, r, rs, p, v, ws->pf2, ws->vf2, t,
-          rel_tol, max_iter - done_it, 0, g_poll, kinds);
+        inner_jacobi(G, tab, b, x, rel_tol, max_iter - done_it);
         g_learn.jacobi_iters += ws->sc_host->iter;
         ws->sc_host->iter += done_it;
         g_learn.skip_left = g_learn.skip_next;
@@ -1481,8 +1797,6 @@ static void solve_mixed(const PPGeom &G, const DinvTable &tab)
   k_scale2<double><<<eb, 256, 0, st>>>(n, ws->sc, ws->x, b);
   BB_CHECK_LAUNCH();
   /* |b| (close to 1) for the tolerance */
-  int kinds[4] = {BB_K_SPMV_P32, BB_K_SPMV_S32, BB_K_UPDATE_XR32,
-    BB_K_UPDATE_P32};
   double *rn = ws->red + 24;   /* device: |r| of the current outer step */
   double bnorm = 0.;
   int total_inner = 0, outer = 0, converged = 0;
@@ -1529,8 +1843,7 @@ static void solve_mixed(const PPGeom &G, const DinvTable &tab)
         g_learn.skip_left--;
         g_learn.skipped++;
       }
-      bicgstab_run<float>(G, tab, ws->bf, ws->xf, ws->rf, ws->rsf, ws->pf,
-        ws->vf, ws->pf2, ws->vf2, ws->tf, inner_tol, imax, 0, g_poll, kinds);
+      inner_jacobi(G, tab, ws->bf, ws->xf, inner_tol, imax);
       collect_pair(ws);
     }
     total_inner += ws->sc_host->iter + (ws->sc_host->s_conv ? 1 : 0);

@@ -87,6 +87,7 @@ EXTRA_HIP = {
     "bb_prof_reset": (None, []),
     "bb_set_poll_interval": (None, [C.c_int]),
     "bb_set_mixed_params": (None, [C.c_double, C.c_int, C.c_int]),
+    "bb_set_inner_sweeps": (C.c_int, [C.c_int]),
     "bb_set_implicit": (None, [C.c_int]),
     "bb_learn_collect": (None, [C.c_int]),
     "bb_learn_pairs": (C.c_int, []),
@@ -114,7 +115,7 @@ PROF_KINDS = [
     "update_p", "final", "resid64", "correct64", "spmv_p32", "spmv_s32",
     "update_xr32", "update_p32", "project", "update_pressure", "find_dt",
     "cages", "part_bc", "quadrature", "move_parts", "misc", "unet",
-    "unet_conv0", "fspmv", "fvec",
+    "unet_conv0", "fspmv", "fvec", "sweep_a32", "sweep_b32",
 ]
 
 

@@ -393,6 +393,13 @@ void bb_set_poll_interval(int n);
  * caps (values <= 0 leave a parameter unchanged). */
 void bb_set_mixed_params(real inner_tol, int inner_max, int outer_max);
 
+/* Mixed and learned modes: the Jacobi BiCGSTAB of the FP32 inner solve in
+ * three sweeps per iteration (3, the default: the textbook iteration) or
+ * two (2: the update sweep folded into the next iteration's first sweep,
+ * its dots taken by linearity in the second; 58 instead of 71 B/cell).
+ * Returns the previous setting. */
+int bb_set_inner_sweeps(int n);
+
 /* Per-kernel event timing on the library stream.  Kinds index the kernel
  * families (predictor, BCs, Poisson set-up, the four BiCGSTAB sweeps in
  * FP64 and FP32, refinement residual and correction, projection, pressure
@@ -408,6 +415,7 @@ enum {
   BB_PROF_PROJECT, BB_PROF_UPDATE_PRESSURE, BB_PROF_FIND_DT, BB_PROF_CAGES,
   BB_PROF_PART_BC, BB_PROF_QUADRATURE, BB_PROF_MOVE_PARTS, BB_PROF_MISC,
   BB_PROF_UNET, BB_PROF_UNET_CONV0, BB_PROF_FSPMV, BB_PROF_FVEC,
+  BB_PROF_SWEEP_A32, BB_PROF_SWEEP_B32,
   BB_PROF_NKINDS
 };
 void bb_prof_enable(int on);

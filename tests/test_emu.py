@@ -1,9 +1,10 @@
 """The HIP kernels, built for the host by tests/emu (g++ with
 AddressSanitizer and a bounds-checked HIP emulation), run one full step of
 a small particle case and of a wall-bounded case on the CPU and agree with
-the oracle.  An out-of-bounds index, a host pointer passed to a kernel or
-an invalid launch shape aborts the child process with a report, so this
-guards every kernel change before it reaches a GPU.  The full GPU suite
+the oracle, and so does the mixed form's two-sweep inner solve.  An
+out-of-bounds index, a host pointer passed to a kernel or an invalid
+launch shape aborts the child process with a report, so this guards every
+kernel change before it reaches a GPU.  The full GPU suite
 runs the same way with BB_EMU=1 (see tests/conftest.py)."""
 import os
 import subprocess
@@ -46,6 +47,23 @@ for name, case in (
         assert e < 1e-8, (name, f, e)
     for f in ("phase", "phase_shell", "flag_u", "flag_v", "flag_w"):
         assert np.array_equal(s.array(f), o.array(f)), (name, f)
+    # the mixed form's two-sweep inner solve, cold-started, against the
+    # FP64 oracle's solve of the same system
+    s.hip.cuda_U_star_2()
+    s.hip.cuda_dom_BC_star()
+    s.pull()
+    s.array("phi")[:] = 0.0
+    s.push()
+    o = OracleState.from_sim(s)
+    s.set_solver(1)
+    s.hip.bb_set_inner_sweeps(2)
+    s.hip.cuda_PP_bicgstab(0)
+    s.hip.bb_set_inner_sweeps(3)
+    s.set_solver(0)
+    s.pull()
+    o.call("PP_bicgstab")
+    e = rel(s.array("phi"), o.array("phi"))
+    assert e < 1e-8, (name, "two-sweep phi", e)
     s.close()
     print("emulated step", name, "ok", flush=True)
 """

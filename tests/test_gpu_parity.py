@@ -144,6 +144,33 @@ def test_pp_bicgstab_mixed(warmed):
     print("PP mixed: phi rel err %.3e" % e["phi"])
 
 
+def test_pp_bicgstab_mixed_two_sweep(warmed):
+    """The mixed form with its FP32 inner solve in two sweeps per iteration
+    (bb_set_inner_sweeps(2)): the same true-residual test, so the same
+    TOL_MIXED bound against the FP64 oracle."""
+    _, s = warmed
+    s.hip.cuda_U_star_2()
+    s.hip.cuda_dom_BC_star()
+    s.pull()
+    s.array("phi")[:] = 0.0
+    s.push()
+    s.set_solver(1)
+    old = s.hip.bb_set_inner_sweeps(2)
+    try:
+        o = _stage(s, lambda: s.hip.cuda_PP_bicgstab(0),
+                   lambda o: o.call("PP_bicgstab"))
+    finally:
+        s.hip.bb_set_inner_sweeps(old)
+        s.set_solver(0)
+    st = s.solve_stats()
+    print("PP mixed, two sweeps: gpu outer %d inner %d resid %.3e | oracle "
+          "iters %d" % (st.outer_steps, st.iterations, st.residual,
+                        o.s.last_iters))
+    assert st.converged and o.s.last_converged
+    e = compare_fields(s, o, ("phi",), TOL_MIXED)
+    print("PP mixed, two sweeps: phi rel err %.3e" % e["phi"])
+
+
 def test_project(warmed):
     _, s = warmed
     o = _stage(s, s.hip.cuda_project, lambda o: o.call("project"))
