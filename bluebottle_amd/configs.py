@@ -362,3 +362,25 @@ def read_case(root, noise=1e-3, seed=1):
                 seed=seed, duration=gs("duration"),
                 interaction_length_ratio=gs("interactionLengthRatio"),
                 particles=parts)
+
+
+CONFIGS = {"config1": config1, "config2": config2, "config3": config3,
+           "config5": config5}
+
+
+def main(argv=None):
+    """python -m bluebottle_amd.configs NAME ROOT: write the seeded inputs of
+    configuration NAME (config1, config2, config3, config5) as
+    ROOT/input/flow.config and ROOT/input/part.config."""
+    import sys
+    args = sys.argv[1:] if argv is None else argv
+    if len(args) != 2 or args[0] not in CONFIGS:
+        print("usage: python -m bluebottle_amd.configs {%s} ROOT"
+              % ",".join(sorted(CONFIGS)), file=sys.stderr)
+        return 2
+    write_case(CONFIGS[args[0]](), args[1])
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -131,6 +131,28 @@ def test_case_round_trip(which, tmp_path):
                                 tuple(a[k]) == tuple(b[k])), k
 
 
+@pytest.mark.parametrize("which", ["config1", "config2", "config3",
+                                   "config5"])
+def test_committed_inputs_are_the_seeded_configurations(which, tmp_path):
+    """bluebottle_amd/inputs/<config>/input holds the seeded set-up of each
+    benchmark configuration in the reference's grammar, byte for byte what
+    `python -m bluebottle_amd.configs <config> ROOT` writes, and the
+    driver's readers parse it back to the same case."""
+    import os
+    root = os.path.join(os.path.dirname(configs.__file__), "inputs", which)
+    assert configs.main([which, str(tmp_path)]) == 0
+    for name in ("flow.config", "part.config"):
+        with open(os.path.join(root, "input", name), "rb") as f:
+            committed = f.read()
+        assert committed == (tmp_path / "input" / name).read_bytes(), name
+    case = configs.CONFIGS[which]()
+    back = configs.read_case(root, noise=case.noise, seed=case.seed)
+    assert len(back.particles) == len(case.particles)
+    for p, q in zip(case.particles, back.particles):
+        assert (p.x, p.y, p.z, p.r) == (q.x, q.y, q.z, q.r)
+    assert back.n == case.n and back.bc_u == case.bc_u
+
+
 # ---------------------------------------------------------------------------
 # restart
 # ---------------------------------------------------------------------------
