@@ -320,10 +320,13 @@ def read_case(root, noise=1e-3, seed=1):
     b = runtime.bc()
 
     def spec(comp):
+        # the grammar gives a value only to Dirichlet (and precursor) faces;
+        # the globals keep whatever an earlier read left for the others
         out = []
         for face in _FACES:
             t = getattr(b, comp + face)
-            v = 0.0 if comp == "p" else getattr(b, comp + face + "Dm")
+            v = getattr(b, comp + face + "Dm") if comp != "p" and t in (
+                abi.DIRICHLET, abi.PRECURSOR) else 0.0
             out.append((t, v))
         return tuple(out)
 

@@ -115,6 +115,9 @@ void bb_free_workspace(void);
 void bb_learn_free(void);
 /* Crank-Nicolson solver workspace (bb_helmholtz.hip) */
 void bb_free_helmholtz(void);
+/* precursor inflow planes (bb_precursor.hip), with cuda_dom_malloc/free */
+void bb_alloc_planes(void);
+void bb_free_planes(void);
 /* Particle workspace sized for nparts and the current grid. */
 void bb_ensure_particle_workspace(void);
 void bb_invalidate_face_lists(void);

@@ -201,6 +201,15 @@ extern "C" void cuda_U_star_2(void)
  * then the high face of every field in the batch over the full ghosted
  * transverse extent, matching the reference's W,E,S,N,B,T kernel order.
  * ------------------------------------------------------------------- */
+/* A PRECURSOR face reads its values from inflow planes: the normal
+ * component from one plane, as a Dirichlet value per position; a
+ * tangential component sets the ghost and the first interior value (low
+ * face) or the last interior value and the ghost (high face) from two
+ * planes.  BC_PRECURSOR_AT_LO on a high face applies the low face's
+ * normal-component fill there instead (the reference's cuda_dom_BC_star
+ * calls its south kernel for the north face of v). */
+#define BC_PRECURSOR_AT_LO 103
+
 struct BCField {
   double *f;
   int s1, s2;          /* strides */
@@ -208,6 +217,7 @@ struct BCField {
   int normal;          /* staggered axis, -1 for cell-centred */
   int type[6];
   double val[6];
+  const double *pl[6][2];   /* inflow planes of PRECURSOR faces */
 };
 
 struct BCBatch {
@@ -216,12 +226,26 @@ struct BCBatch {
 };
 
 __device__ inline void bc_apply(double *F, long str, int lo, int type,
-  int normal, double bcv, int s, int e, int sb, int eb)
+  int normal, double bcv, int s, int e, int sb, int eb,
+  const double *const *pl, long P)
 {
   /* ghost values are bit-identical to the reference's separately rounded
    * products and sums */
 #pragma clang fp contract(off)
-  if(type == PERIODIC) {
+  if(type == PRECURSOR) {
+    if(normal) {
+      const double b = pl[0][P];
+      if(lo) { F[sb*str] = 2. * b - F[(s+1)*str]; F[s*str] = b; }
+      else   { F[(eb-1)*str] = 2. * b - F[(e-2)*str]; F[(e-1)*str] = b; }
+    } else {
+      const double b0 = pl[0][P], b1 = pl[1][P];
+      if(lo) { F[sb*str] = b0; F[s*str] = b1; }
+      else   { F[(e-1)*str] = b0; F[(eb-1)*str] = b1; }
+    }
+  } else if(type == BC_PRECURSOR_AT_LO) {
+    const double b = pl[0][P];
+    F[sb*str] = 2. * b - F[(s+1)*str]; F[s*str] = b;
+  } else if(type == PERIODIC) {
     if(normal) {
       if(lo) { F[sb*str] = F[(e-2)*str]; F[s*str] = F[(e-1)*str]; }
       else   { F[(eb-1)*str] = F[(s+1)*str]; F[(e-1)*str] = F[s*str]; }
@@ -256,6 +280,8 @@ struct BCAxisField {
   int normal;          /* 1 when the axis is the field's staggered axis */
   int type_lo, type_hi;
   double val_lo, val_hi;
+  const double *pl_lo[2], *pl_hi[2];   /* planes of PRECURSOR faces */
+  long p1, p2;         /* plane strides of the two transverse axes */
 };
 
 struct BCAxisBatch {
@@ -273,14 +299,49 @@ __global__ void __launch_bounds__(256) k_bc_axis(BCAxisBatch B)
     const BCAxisField &fd = B.fld[q];
     if(t1 >= fd.n1 || t2 >= fd.n2) continue;
     double *F = fd.f + (long) t1 * fd.s1 + (long) t2 * fd.s2;
+    const long P = (long) t1 * fd.p1 + (long) t2 * fd.p2;
     /* along the axis: sb = 0, s = 1, e = nax-1, eb = nax */
     const int s = 1, e = fd.nax - 1, sb = 0, eb = fd.nax;
-    bc_apply(F, fd.str, 1, fd.type_lo, fd.normal, fd.val_lo, s, e, sb, eb);
-    bc_apply(F, fd.str, 0, fd.type_hi, fd.normal, fd.val_hi, s, e, sb, eb);
+    bc_apply(F, fd.str, 1, fd.type_lo, fd.normal, fd.val_lo, s, e, sb, eb,
+      fd.pl_lo, P);
+    bc_apply(F, fd.str, 0, fd.type_hi, fd.normal, fd.val_hi, s, e, sb, eb,
+      fd.pl_hi, P);
   }
 }
 
-static BCField bc_field(double *f, char which)
+/* the inflow planes the reference's cuda_dom_BC (star = 0) and
+ * cuda_dom_BC_star (star = 1) pass for a PRECURSOR face of component
+ * `which`; its particulars are kept: cuda_dom_BC gives the west face of v
+ * v_WE_W for both planes, cuda_dom_BC_star fills the north face of v with
+ * the south kernel.  Departure: cuda_dom_BC's north-face w call writes into
+ * u at w's indices (out of u's bounds unless the grids coincide); here it
+ * fills w. */
+static void precursor_planes(BCField &fd, char which, int star)
+{
+  for(int q = 0; q < 6; q++) fd.pl[q][0] = fd.pl[q][1] = NULL;
+  if(which == 'u') {
+    fd.pl[0][0] = fd.pl[1][0] = _u_WE[0];
+    fd.pl[2][0] = fd.pl[3][0] = _u_SN_S[0];
+    fd.pl[2][1] = fd.pl[3][1] = _u_SN_N[0];
+    fd.pl[4][0] = fd.pl[5][0] = _u_BT_B[0];
+    fd.pl[4][1] = fd.pl[5][1] = _u_BT_T[0];
+  } else if(which == 'v') {
+    fd.pl[0][0] = _v_WE_W[0]; fd.pl[0][1] = star ? _v_WE_E[0] : _v_WE_W[0];
+    fd.pl[1][0] = _v_WE_W[0]; fd.pl[1][1] = _v_WE_E[0];
+    fd.pl[2][0] = fd.pl[3][0] = _v_SN[0];
+    fd.pl[4][0] = fd.pl[5][0] = _v_BT_B[0];
+    fd.pl[4][1] = fd.pl[5][1] = _v_BT_T[0];
+    if(star && fd.type[3] == PRECURSOR) fd.type[3] = BC_PRECURSOR_AT_LO;
+  } else if(which == 'w') {
+    fd.pl[0][0] = fd.pl[1][0] = _w_WE_W[0];
+    fd.pl[0][1] = fd.pl[1][1] = _w_WE_E[0];
+    fd.pl[2][0] = fd.pl[3][0] = _w_SN_S[0];
+    fd.pl[2][1] = fd.pl[3][1] = _w_SN_N[0];
+    fd.pl[4][0] = fd.pl[5][0] = _w_BT[0];
+  }
+}
+
+static BCField bc_field(double *f, char which, int star = 0)
 {
   BCField fd;
   const dom_struct *d = &dom[0];
@@ -312,16 +373,18 @@ static BCField bc_field(double *f, char which)
       for(int q = 0; q < 6; q++) { fd.type[q] = t[q]; fd.val[q] = vv[q]; }
       break; }
   }
-  /* faces of a subdomain with a neighbour are not external; PRECURSOR
-   * faces (turbulent inflow planes) are outside this library's scope */
+  /* faces of a subdomain with a neighbour are not external; the pressure
+   * has no PRECURSOR type */
   int ext[6] = {d->W, d->E, d->S, d->N, d->B, d->T};
   for(int q = 0; q < 6; q++) {
     if(ext[q] != -1) fd.type[q] = -1;
-    if(fd.type[q] == PRECURSOR) {
-      fprintf(stderr, "bluebottle_hip: PRECURSOR boundaries are not supported\n");
+    if(fd.type[q] == PRECURSOR && which == 'p') {
+      fprintf(stderr, "bluebottle_hip: a PRECURSOR pressure boundary is not "
+        "defined\n");
       exit(EXIT_FAILURE);
     }
   }
+  precursor_planes(fd, which, star);
   return fd;
 }
 
@@ -339,6 +402,15 @@ static BCAxisField bc_axis_field(const BCField &fd, int axis)
   a.normal = fd.normal == axis;
   a.type_lo = fd.type[2*axis]; a.type_hi = fd.type[2*axis+1];
   a.val_lo = fd.val[2*axis]; a.val_hi = fd.val[2*axis+1];
+  for(int q = 0; q < 2; q++) {
+    a.pl_lo[q] = fd.pl[2*axis][q];
+    a.pl_hi[q] = fd.pl[2*axis+1][q];
+  }
+  /* plane layouts: W/E j + k*jnb, S/N k + i*knb, B/T i + j*inb, with
+   * (t1, t2) = (j, k), (i, k), (i, j) */
+  if(axis == 0) { a.p1 = 1; a.p2 = fd.n[1]; }
+  else if(axis == 1) { a.p1 = fd.n[2]; a.p2 = 1; }
+  else { a.p1 = 1; a.p2 = fd.n[0]; }
   return a;
 }
 
@@ -380,9 +452,9 @@ extern "C" void cuda_dom_BC_star(void)
   bb_ensure_context();
   BCBatch B;
   B.nf = 3;
-  B.fld[0] = bc_field(_u_star[0], 'u');
-  B.fld[1] = bc_field(_v_star[0], 'v');
-  B.fld[2] = bc_field(_w_star[0], 'w');
+  B.fld[0] = bc_field(_u_star[0], 'u', 1);
+  B.fld[1] = bc_field(_v_star[0], 'v', 1);
+  B.fld[2] = bc_field(_w_star[0], 'w', 1);
   bc_launch(B);
 }
 

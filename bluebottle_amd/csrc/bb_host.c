@@ -39,6 +39,12 @@ real *conv_u, *conv_v, *conv_w;
 real **_conv_u, **_conv_v, **_conv_w;
 real **_u_star, **_v_star, **_w_star;
 real *u_star, *v_star, *w_star;
+real *u_WE, *u_SN_S, *u_SN_N, *u_BT_B, *u_BT_T;
+real *v_WE_W, *v_WE_E, *v_SN, *v_BT_B, *v_BT_T;
+real *w_WE_W, *w_WE_E, *w_SN_S, *w_SN_N, *w_BT;
+real **_u_WE, **_u_SN_S, **_u_SN_N, **_u_BT_B, **_u_BT_T;
+real **_v_WE_W, **_v_WE_E, **_v_SN, **_v_BT_B, **_v_BT_T;
+real **_w_WE_W, **_w_WE_E, **_w_SN_S, **_w_SN_N, **_w_BT;
 real **_rhs_p;
 real duration, ttime, vel_tDelay, p_tDelay, g_tDelay;
 real dt, dt0, CFL;
@@ -180,6 +186,25 @@ int bb_host_domain(real xs, real xe, int xn, real ys, real ye, int yn,
   conv_w = zalloc(fz, sizeof(real));
   u_star = zalloc(fx, sizeof(real)); v_star = zalloc(fy, sizeof(real));
   w_star = zalloc(fz, sizeof(real));
+  /* the precursor inflow planes (src/domain.c:2058-2087) */
+  {
+    const grid_info *X = &Dom.Gfx, *Y = &Dom.Gfy, *Z = &Dom.Gfz;
+    u_WE = zalloc((size_t) X->jnb * X->knb, sizeof(real));
+    u_SN_S = zalloc((size_t) X->inb * X->knb, sizeof(real));
+    u_SN_N = zalloc((size_t) X->inb * X->knb, sizeof(real));
+    u_BT_B = zalloc((size_t) X->inb * X->jnb, sizeof(real));
+    u_BT_T = zalloc((size_t) X->inb * X->jnb, sizeof(real));
+    v_WE_W = zalloc((size_t) Y->jnb * Y->knb, sizeof(real));
+    v_WE_E = zalloc((size_t) Y->jnb * Y->knb, sizeof(real));
+    v_SN = zalloc((size_t) Y->inb * Y->knb, sizeof(real));
+    v_BT_B = zalloc((size_t) Y->inb * Y->jnb, sizeof(real));
+    v_BT_T = zalloc((size_t) Y->inb * Y->jnb, sizeof(real));
+    w_WE_W = zalloc((size_t) Z->jnb * Z->knb, sizeof(real));
+    w_WE_E = zalloc((size_t) Z->jnb * Z->knb, sizeof(real));
+    w_SN_S = zalloc((size_t) Z->inb * Z->knb, sizeof(real));
+    w_SN_N = zalloc((size_t) Z->inb * Z->knb, sizeof(real));
+    w_BT = zalloc((size_t) Z->inb * Z->jnb, sizeof(real));
+  }
   phase = zalloc(cc, sizeof(int)); phase_shell = zalloc(cc, sizeof(int));
   flag_u = zalloc(fx, sizeof(int)); flag_v = zalloc(fy, sizeof(int));
   flag_w = zalloc(fz, sizeof(int));
@@ -292,8 +317,12 @@ void bb_host_free(void)
     flag_u, flag_v, flag_w, parts, dom, pnm_re, pnm_im, phinm_re, phinm_im,
     chinm_re, chinm_im, pnm_re0, pnm_im0, phinm_re0, phinm_im0, chinm_re0,
     chinm_im0, pnm_re00, pnm_im00, phinm_re00, phinm_im00, chinm_re00,
-    chinm_im00};
+    chinm_im00, u_WE, u_SN_S, u_SN_N, u_BT_B, u_BT_T, v_WE_W, v_WE_E, v_SN,
+    v_BT_B, v_BT_T, w_WE_W, w_WE_E, w_SN_S, w_SN_N, w_BT};
   for(size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+  u_WE = u_SN_S = u_SN_N = u_BT_B = u_BT_T = NULL;
+  v_WE_W = v_WE_E = v_SN = v_BT_B = v_BT_T = NULL;
+  w_WE_W = w_WE_E = w_SN_S = w_SN_N = w_BT = NULL;
   p0 = p = phi = u = u0 = v = v0 = w = w0 = NULL;
   f_x = f_y = f_z = NULL;
   diff0_u = diff0_v = diff0_w = conv0_u = conv0_v = conv0_w = NULL;

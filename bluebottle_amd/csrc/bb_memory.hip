@@ -272,6 +272,7 @@ extern "C" void cuda_dom_malloc(void)
   _rhs_p[0] = (real *) dmalloc((size_t) dom[0].Gcc.s3 * sizeof(real));
   BB_CHECK(hipMemsetAsync(_rhs_p[0], 0, (size_t) dom[0].Gcc.s3 * sizeof(real),
     g_bb.stream));
+  bb_alloc_planes();
   bb_ensure_workspace();
   BB_CHECK(hipStreamSynchronize(g_bb.stream));
 }
@@ -304,6 +305,7 @@ extern "C" void cuda_dom_free(void)
   }
   if(_rhs_p) { dfree(_rhs_p[0]); free(_rhs_p); _rhs_p = NULL; }
   if(_dom) { dfree(_dom[0]); free(_dom); _dom = NULL; }
+  bb_free_planes();
   bb_learn_free();
   bb_free_workspace();
   bb_free_helmholtz();

@@ -1,7 +1,7 @@
 /*
  * bb_outside.hip -- entry points the reference driver links that lie
- * outside this library's path: the precursor (turbulent inflow) planes and
- * forcing (src/precursor.c, src/bluebottle.c:956-1044), the -DTEST checks
+ * outside this library's path: the precursor domain's turbulence forcing
+ * (src/precursor.c, src/bluebottle.c:956-1044), the -DTEST checks
  * of src/cuda_testing.cu, and a few helpers no caller uses.  They exist so
  * that the unchanged src/bluebottle.c links against this library; each one
  * prints which path it belongs to and exits, as a missing feature should
@@ -22,11 +22,9 @@ static void outside(const char *name, const char *what)
 #define PRECURSOR_PATH "precursor inflow"
 #define TEST_PATH "-DTEST check"
 
-/* src/bluebottle.h:2411, 2447, 2877, 2926 */
-extern "C" void cuda_dom_turb_planes_push(int *) { outside(__func__, PRECURSOR_PATH); }
-extern "C" void cuda_dom_turb_planes_pull(int *) { outside(__func__, PRECURSOR_PATH); }
+/* src/bluebottle.h:2877 (the precursor domain's turbulence forcing; the
+ * planes themselves are in bb_precursor.hip) */
 extern "C" void cuda_compute_turb_forcing(void) { outside(__func__, PRECURSOR_PATH); }
-extern "C" void cuda_yank_turb_planes(int *, real *, real *) { outside(__func__, PRECURSOR_PATH); }
 
 /* src/bluebottle.h:2616, 2628, 2753, 2766, 2780, 2792 */
 extern "C" void cuda_U_star_test_exp(void) { outside(__func__, TEST_PATH); }

@@ -43,6 +43,9 @@ ENTRY_POINTS = {
     "cuda_part_p_fill": (None, []), "cuda_store_coeffs": (None, []),
     "cuda_move_parts_sub": (None, []), "cuda_move_parts": (None, []),
     "cuda_parts_internal": (None, []),
+    "cuda_dom_turb_planes_push": (None, [C.POINTER(C.c_int)]),
+    "cuda_dom_turb_planes_pull": (None, [C.POINTER(C.c_int)]),
+    "cuda_yank_turb_planes": (None, [C.POINTER(C.c_int), _PD, _PD]),
     # node tables and node values are device pointers
     "cuda_quad_check_nodes": (None, [C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
     "cuda_quad_interp": (None, [C.c_int, C.c_void_p, C.c_void_p, C.c_int,

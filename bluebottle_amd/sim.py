@@ -248,9 +248,27 @@ class Simulation:
             self.array(comp + "0")[:] = f
 
     # ------------------------------------------------------------------
+    # precursor inflow planes: grid and the axis each is normal to
+    PLANES = {"u_WE": ("u", 0), "u_SN_S": ("u", 1), "u_SN_N": ("u", 1),
+              "u_BT_B": ("u", 2), "u_BT_T": ("u", 2),
+              "v_WE_W": ("v", 0), "v_WE_E": ("v", 0), "v_SN": ("v", 1),
+              "v_BT_B": ("v", 2), "v_BT_T": ("v", 2),
+              "w_WE_W": ("w", 0), "w_WE_E": ("w", 0), "w_SN_S": ("w", 1),
+              "w_SN_N": ("w", 1), "w_BT": ("w", 2)}
+
+    def plane_shape(self, name):
+        """(rows, row length) of a plane: W/E (knb, jnb), S/N (inb, knb),
+        B/T (jnb, inb) of its component's grid."""
+        comp, axis = self.PLANES[name]
+        G = self.grid(comp)
+        return ((G.knb, G.jnb), (G.inb, G.knb), (G.jnb, G.inb))[axis]
+
     def count(self, name):
         if name.startswith(("pnm", "phinm", "chinm")):
             return runtime.get_scalar("coeff_stride", "i") * len(self.case.particles)
+        if name in self.PLANES:
+            a, b = self.plane_shape(name)
+            return a * b
         return self.grid(name).s3b
 
     def array(self, name):

@@ -215,6 +215,15 @@ extern real *conv_u, *conv_v, *conv_w;
 extern real **_conv_u, **_conv_v, **_conv_w;
 extern real *u_star, *v_star, *w_star;
 extern real **_u_star, **_v_star, **_w_star;
+/* precursor inflow planes (src/bluebottle.c:92-121): W/E planes indexed
+ * j + k*jnb, S/N planes k + i*knb, B/T planes i + j*inb of the
+ * component's grid */
+extern real *u_WE, *u_SN_S, *u_SN_N, *u_BT_B, *u_BT_T;
+extern real *v_WE_W, *v_WE_E, *v_SN, *v_BT_B, *v_BT_T;
+extern real *w_WE_W, *w_WE_E, *w_SN_S, *w_SN_N, *w_BT;
+extern real **_u_WE, **_u_SN_S, **_u_SN_N, **_u_BT_B, **_u_BT_T;
+extern real **_v_WE_W, **_v_WE_E, **_v_SN, **_v_BT_B, **_v_BT_T;
+extern real **_w_WE_W, **_w_WE_E, **_w_SN_S, **_w_SN_N, **_w_BT;
 extern real **_rhs_p;
 extern real duration, ttime, vel_tDelay, p_tDelay, g_tDelay;
 extern real dt, dt0, CFL;

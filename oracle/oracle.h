@@ -63,10 +63,15 @@ typedef struct orc_state {
    * instead of the explicit predictor, no diff0 history, dt without the
    * viscous limit and capped at 1.1 dt) */
   int implicit;
+  /* precursor inflow planes, in the order of orc_plane_names (layouts:
+   * W/E j + k*jnb, S/N k + i*knb, B/T i + j*inb of the component's grid) */
+  real *plane[15];
 } orc_state;
 
 /* Names of coeff[] entries, in order. */
 extern const char *orc_coeff_names[18];
+/* Names of plane[] entries, in order (the reference's globals). */
+extern const char *orc_plane_names[15];
 
 /* --- state management (oracle_state.c) --- */
 orc_state *orc_create(const dom_struct *Dom, int nparts, int coeff_stride);
@@ -83,8 +88,11 @@ void orc_set_threads(int n);
 /* src/bluebottle_kernel.cu:1504-2003 via src/cuda_bluebottle.cu:2930. */
 void orc_U_star_2(orc_state *s);
 /* src/cuda_bluebottle.cu:1549-3420 with the BC_* kernels at
- * src/bluebottle_kernel.cu:26-1248 (PRECURSOR faces unsupported). */
+ * src/bluebottle_kernel.cu:26-1248 (PRECURSOR faces from s->plane). */
 void orc_dom_BC(orc_state *s);
+/* src/cuda_bluebottle.cu:4556-4820, src/bluebottle_kernel.cu:3619-3851 */
+void orc_yank_turb_planes(orc_state *s, const int *bc_flow_configs,
+  const real *pos, const real *vel);
 void orc_dom_BC_star(orc_state *s);
 void orc_dom_BC_phi(orc_state *s);
 void orc_dom_BC_p(orc_state *s);

@@ -176,8 +176,16 @@ void orc_U_star_2(orc_state *s)
  *   normal: 1 if the field is staggered along `axis` (u in x, v in y,
  *   w in z), else 0 (cell-centred along the axis, also p).
  * ------------------------------------------------------------------- */
+/* PRECURSOR faces (the BC_*_T kernels, src/bluebottle_kernel.cu:228-1247)
+ * read inflow planes pl0 (and pl1 for a tangential component), indexed
+ * j + k*jnb (W/E), k + i*knb (S/N), i + j*inb (B/T).  PRECURSOR_AT_LO on a
+ * high face applies the low face's normal-component kernel, as
+ * cuda_dom_BC_star does for the north face of v (src/cuda_bluebottle.cu,
+ * its BC_v_S_T call in the north-face switch). */
+#define PRECURSOR_AT_LO 103
+
 static void bc_face(real *f, grid_info G, int axis, int hi, int type,
-  int normal, real bcv)
+  int normal, real bcv, const real *pl0, const real *pl1)
 {
   int s1 = 1, s2, s3;                 /* strides along axis, t1, t2 */
   int n1, n2;                         /* transverse extents incl. ghosts */
@@ -197,8 +205,20 @@ static void bc_face(real *f, grid_info G, int axis, int hi, int type,
   for(int t2 = 0; t2 < n2; t2++)
   for(int t1 = 0; t1 < n1; t1++) {
     real *F = f + (long) t1 * s2 + (long) t2 * s3;
+    const long P = axis == 0 ? t1 + (long) t2 * G.jnb
+      : axis == 1 ? t2 + (long) t1 * G.knb : t1 + (long) t2 * G.inb;
 #define A(n) F[(long) (n) * str]
-    if(type == PERIODIC) {
+    if(type == PRECURSOR) {
+      if(normal) {
+        if(!hi) { A(sb) = 2. * pl0[P] - A(s+1); A(s) = pl0[P]; }
+        else    { A(eb-1) = 2. * pl0[P] - A(e-2); A(e-1) = pl0[P]; }
+      } else {
+        if(!hi) { A(sb) = pl0[P]; A(s) = pl1[P]; }
+        else    { A(e-1) = pl0[P]; A(eb-1) = pl1[P]; }
+      }
+    } else if(type == PRECURSOR_AT_LO) {
+      A(sb) = 2. * pl0[P] - A(s+1); A(s) = pl0[P];
+    } else if(type == PERIODIC) {
       if(normal) {
         if(!hi) { A(sb) = A(e-2); A(s) = A(e-1); }
         else    { A(eb-1) = A(s+1); A(e-1) = A(s); }
@@ -252,7 +272,26 @@ static void bc_table(const BC *b, char field, int type[6], real val[6])
   }
 }
 
-static void bc_field(const orc_state *s, real *f, char field)
+/* The planes the reference passes for each PRECURSOR face of a component
+ * (plane indices into orc_plane_names; -1 none), from cuda_dom_BC
+ * (star = 0) and cuda_dom_BC_star (star = 1), src/cuda_bluebottle.cu:
+ * 1549-2298 and 2300-3030.  Kept: cuda_dom_BC passes v_WE_W for both
+ * planes of v's west face; cuda_dom_BC_star fills v's north face with the
+ * south kernel.  Departure: cuda_dom_BC's north-face call for w writes
+ * into u at w's indices (out of u's bounds unless the grids coincide);
+ * here it fills w. */
+static void precursor_planes(int comp, int star, int pl[6][2])
+{
+  static const int U[6][2] = {{0, -1}, {0, -1}, {1, 2}, {1, 2}, {3, 4}, {3, 4}};
+  static const int V[6][2] = {{5, 5}, {5, 6}, {7, -1}, {7, -1}, {8, 9}, {8, 9}};
+  static const int W[6][2] = {{10, 11}, {10, 11}, {12, 13}, {12, 13},
+    {14, -1}, {14, -1}};
+  const int (*T)[2] = comp == 'u' ? U : comp == 'v' ? V : W;
+  for(int f = 0; f < 6; f++) { pl[f][0] = T[f][0]; pl[f][1] = T[f][1]; }
+  if(comp == 'v' && star) pl[0][1] = 6;
+}
+
+static void bc_field(const orc_state *s, real *f, char field, int star)
 {
   const dom_struct *d = &s->dom;
   grid_info G = field == 'p' ? d->Gcc : field == 'u' ? d->Gfx
@@ -261,31 +300,142 @@ static void bc_field(const orc_state *s, real *f, char field)
   int type[6];
   real val[6];
   bc_table(&s->bc, field, type, val);
+  int pl[6][2] = {{-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}};
+  if(field != 'p') precursor_planes(field, star, pl);
+  if(field == 'v' && star && type[3] == PRECURSOR) type[3] = PRECURSOR_AT_LO;
   int ext[6] = {d->W, d->E, d->S, d->N, d->B, d->T};
   for(int face = 0; face < 6; face++) {
     if(ext[face] != -1) continue;
     int axis = face / 2, hi = face % 2;
-    bc_face(f, G, axis, hi, type[face], axis == normal_axis, val[face]);
+    const real *p0 = pl[face][0] >= 0 ? s->plane[pl[face][0]] : NULL;
+    const real *p1 = pl[face][1] >= 0 ? s->plane[pl[face][1]] : NULL;
+    bc_face(f, G, axis, hi, type[face], axis == normal_axis, val[face], p0, p1);
   }
 }
 
 void orc_dom_BC(orc_state *s)
 {
-  bc_field(s, s->p, 'p');
-  bc_field(s, s->u, 'u');
-  bc_field(s, s->v, 'v');
-  bc_field(s, s->w, 'w');
+  bc_field(s, s->p, 'p', 0);
+  bc_field(s, s->u, 'u', 0);
+  bc_field(s, s->v, 'v', 0);
+  bc_field(s, s->w, 'w', 0);
 }
 
 void orc_dom_BC_star(orc_state *s)
 {
-  bc_field(s, s->u_star, 'u');
-  bc_field(s, s->v_star, 'v');
-  bc_field(s, s->w_star, 'w');
+  bc_field(s, s->u_star, 'u', 1);
+  bc_field(s, s->v_star, 'v', 1);
+  bc_field(s, s->w_star, 'w', 1);
 }
 
-void orc_dom_BC_phi(orc_state *s) { bc_field(s, s->phi, 'p'); }
-void orc_dom_BC_p(orc_state *s) { bc_field(s, s->p, 'p'); }
+void orc_dom_BC_phi(orc_state *s) { bc_field(s, s->phi, 'p', 0); }
+void orc_dom_BC_p(orc_state *s) { bc_field(s, s->p, 'p', 0); }
+
+/* ---------------------------------------------------------------------
+ * Plane extraction of the precursor domain: cuda_yank_turb_planes
+ * (src/cuda_bluebottle.cu:4556-4820) with the yank_* kernels
+ * (src/bluebottle_kernel.cu:3619-3851).  Along the plane's normal axis the
+ * cell index of pos (one periodic wrap), then linear interpolation of the
+ * face values: the normal component at pos plus vel; a tangential one
+ * half a cell either side of pos.  Kept: yank_w_BT scales by 1/dx;
+ * yank_w_SN's high plane is the slope term alone and is also written
+ * into w (`plane_n = w[M] = ...`); the north-face w call passes w_SN_N
+ * for both planes.  Departure: yank_u_SN bounds k by Gfx.inb (one row
+ * past the arrays when inb > knb); here k < min(inb, knb).  Plane
+ * positions outside the domain are clamped to the grid (the reference
+ * reads out of bounds there).
+ * ------------------------------------------------------------------- */
+static void yank(real *f, grid_info G, int axis, int normal, real *lo,
+  real *hi, real pos, real vel, real h, real xs, real ddh, int write_back,
+  int n2_cap)
+{
+  long str, s1, s2;
+  int n1, n2, start, end, nb;
+  if(axis == 0) {
+    str = 1; s1 = G.s1b; s2 = G.s2b; n1 = G.jnb; n2 = G.knb;
+    start = G.is; end = G.ie; nb = G.inb;
+  } else if(axis == 1) {
+    str = G.s1b; s1 = 1; s2 = G.s2b; n1 = G.inb; n2 = G.knb;
+    start = G.js; end = G.je; nb = G.jnb;
+  } else {
+    str = G.s2b; s1 = 1; s2 = G.s1b; n1 = G.inb; n2 = G.jnb;
+    start = G.ks; end = G.ke; nb = G.knb;
+  }
+  if(n2_cap > 0 && n2_cap < n2) n2 = n2_cap;
+  for(int t2 = 0; t2 < n2; t2++)
+  for(int t1 = 0; t1 < n1; t1++) {
+    double q = floor((pos - xs) * ddh) + DOM_BUF;
+    if(q < start) q += nb;
+    if(q > end - 1) q -= nb;
+    const double lo_i = normal ? 0. : 1.;
+    const int i = (int) fmin(fmax(q, lo_i), (double) (nb - 2));
+    real *F = f + (long) t1 * s1 + (long) t2 * s2;
+    const long P = axis == 0 ? t1 + (long) t2 * G.jnb
+      : axis == 1 ? t2 + (long) t1 * G.knb : t1 + (long) t2 * G.inb;
+    if(normal) {
+      real xx = (i - DOM_BUF) * h + xs;
+      real W = F[i * str], E = F[(i + 1) * str];
+      real d = (E - W) * ddh;
+      lo[P] = W + d * (pos - xx) + vel;
+    } else {
+      real xw = (i - DOM_BUF - 0.5) * h + xs;
+      real xe = (i - DOM_BUF + 0.5) * h + xs;
+      real W = F[(i - 1) * str], M = F[i * str], E = F[(i + 1) * str];
+      real dw = (M - W) * ddh, de = (E - M) * ddh;
+      lo[P] = W + dw * (pos - 0.5 * h - xw);
+      if(write_back) { F[i * str] = de * (pos + 0.5 * h - xe); hi[P] = F[i * str]; }
+      else hi[P] = M + de * (pos + 0.5 * h - xe);
+    }
+  }
+}
+
+void orc_yank_turb_planes(orc_state *s, const int *c, const real *pos,
+  const real *vel)
+{
+  const dom_struct *d = &s->dom;
+  real **P = s->plane;
+  const real ddx = 1. / d->dx, ddy = 1. / d->dy, ddz = 1. / d->dz;
+  if(c[0] == PRECURSOR)
+    yank(s->u, d->Gfx, 0, 1, P[0], NULL, pos[0], vel[0], d->dx, d->xs, ddx, 0, 0);
+  else if(c[1] == PRECURSOR)
+    yank(s->u, d->Gfx, 0, 1, P[0], NULL, pos[0], vel[1], d->dx, d->xs, ddx, 0, 0);
+  if(c[6] == PRECURSOR)
+    yank(s->v, d->Gfy, 0, 0, P[5], P[6], pos[0], vel[0], d->dx, d->xs, ddx, 0, 0);
+  if(c[7] == PRECURSOR)
+    yank(s->v, d->Gfy, 0, 0, P[5], P[6], pos[0], vel[1], d->dx, d->xs, ddx, 0, 0);
+  if(c[12] == PRECURSOR)
+    yank(s->w, d->Gfz, 0, 0, P[10], P[11], pos[0], vel[0], d->dx, d->xs, ddx, 0, 0);
+  if(c[13] == PRECURSOR)
+    yank(s->w, d->Gfz, 0, 0, P[10], P[11], pos[0], vel[1], d->dx, d->xs, ddx, 0, 0);
+  if(c[2] == PRECURSOR)
+    yank(s->u, d->Gfx, 1, 0, P[1], P[2], pos[4], vel[8], d->dy, d->ys, ddy, 0,
+      d->Gfx.inb);
+  else if(c[3] == PRECURSOR)
+    yank(s->u, d->Gfx, 1, 0, P[1], P[2], pos[4], vel[9], d->dy, d->ys, ddy, 0,
+      d->Gfx.inb);
+  if(c[8] == PRECURSOR)
+    yank(s->v, d->Gfy, 1, 1, P[7], NULL, pos[4], vel[8], d->dy, d->ys, ddy, 0, 0);
+  if(c[9] == PRECURSOR)
+    yank(s->v, d->Gfy, 1, 1, P[7], NULL, pos[4], vel[9], d->dy, d->ys, ddy, 0, 0);
+  if(c[14] == PRECURSOR)
+    yank(s->w, d->Gfz, 1, 0, P[12], P[13], pos[4], vel[8], d->dy, d->ys, ddy, 1, 0);
+  if(c[15] == PRECURSOR)
+    yank(s->w, d->Gfz, 1, 0, P[13], P[13], pos[4], vel[9], d->dy, d->ys, ddy, 1, 0);
+  if(c[4] == PRECURSOR)
+    yank(s->u, d->Gfx, 2, 0, P[3], P[4], pos[8], vel[16], d->dz, d->zs, ddz, 0, 0);
+  else if(c[5] == PRECURSOR)
+    yank(s->u, d->Gfx, 2, 0, P[3], P[4], pos[8], vel[17], d->dz, d->zs, ddz, 0, 0);
+  if(c[10] == PRECURSOR)
+    yank(s->v, d->Gfy, 2, 0, P[8], P[9], pos[8], vel[16], d->dz, d->zs, ddz, 0, 0);
+  if(c[11] == PRECURSOR)
+    yank(s->v, d->Gfy, 2, 0, P[8], P[9], pos[8], vel[17], d->dz, d->zs, ddz, 0, 0);
+  if(c[16] == PRECURSOR)
+    yank(s->w, d->Gfz, 2, 1, P[14], NULL, pos[8], vel[16], d->dz, d->zs,
+      1. / d->dx, 0, 0);
+  if(c[17] == PRECURSOR)
+    yank(s->w, d->Gfz, 2, 1, P[14], NULL, pos[8], vel[17], d->dz, d->zs,
+      1. / d->dx, 0, 0);
+}
 
 /* ---------------------------------------------------------------------
  * Solvability: src/cuda_bluebottle.cu:4122-4314 with surf_int_*_copy and

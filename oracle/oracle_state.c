@@ -9,6 +9,22 @@
 #include <string.h>
 #include "oracle.h"
 
+const char *orc_plane_names[15] = {
+  "u_WE", "u_SN_S", "u_SN_N", "u_BT_B", "u_BT_T",
+  "v_WE_W", "v_WE_E", "v_SN", "v_BT_B", "v_BT_T",
+  "w_WE_W", "w_WE_E", "w_SN_S", "w_SN_N", "w_BT"
+};
+
+/* entries of plane q: the transverse ghosted extents of its grid */
+static size_t plane_size(const dom_struct *d, int q)
+{
+  const grid_info *G = q < 5 ? &d->Gfx : q < 10 ? &d->Gfy : &d->Gfz;
+  static const int axis[15] = {0, 1, 1, 2, 2, 0, 0, 1, 2, 2, 0, 0, 1, 1, 2};
+  if(axis[q] == 0) return (size_t) G->jnb * G->knb;
+  if(axis[q] == 1) return (size_t) G->inb * G->knb;
+  return (size_t) G->inb * G->jnb;
+}
+
 const char *orc_coeff_names[18] = {
   "pnm_re", "pnm_im", "phinm_re", "phinm_im", "chinm_re", "chinm_im",
   "pnm_re0", "pnm_im0", "phinm_re0", "phinm_im0", "chinm_re0", "chinm_im0",
@@ -55,6 +71,7 @@ orc_state *orc_create(const dom_struct *Dom, int nparts, int coeff_stride)
   s->parts = zalloc((size_t) nparts, sizeof(part_struct));
   for(int c = 0; c < 18; c++)
     s->coeff[c] = zalloc((size_t) nparts * coeff_stride, sizeof(real));
+  for(int q = 0; q < 15; q++) s->plane[q] = zalloc(plane_size(Dom, q), sizeof(real));
   return s;
 }
 
@@ -69,6 +86,7 @@ void orc_free(orc_state *s)
     s->rhs_p, s->parts};
   for(size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
   for(int c = 0; c < 18; c++) free(s->coeff[c]);
+  for(int q = 0; q < 15; q++) free(s->plane[q]);
   free(s);
 }
 
@@ -101,6 +119,13 @@ void *orc_array(orc_state *s, const char *name, size_t *n, int *is_int)
       *n = tab[i].n;
       *is_int = tab[i].isint;
       return tab[i].ptr;
+    }
+  }
+  for(int q = 0; q < 15; q++) {
+    if(strcmp(orc_plane_names[q], name) == 0) {
+      *n = plane_size(d, q);
+      *is_int = 0;
+      return s->plane[q];
     }
   }
   for(int c = 0; c < 18; c++) {

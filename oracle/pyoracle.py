@@ -51,6 +51,7 @@ class orc_state(C.Structure):
         ("last_iters", C.c_int), ("last_resid", real),
         ("last_converged", C.c_int),
         ("implicit", C.c_int),
+        ("plane", C.POINTER(C.c_double) * 15),
     ]
 
 
@@ -169,6 +170,9 @@ class OracleState:
         s.implicit = 1 if sim.case.implicit else 0
         for name in FIELD_NAMES:
             np.copyto(o.array(name), sim.array(name))
+        for name in getattr(sim, "PLANES", {}):
+            if C.c_void_p.in_dll(host, name).value:
+                np.copyto(o.array(name), sim.array(name))
         if n:
             src = C.POINTER(abi.part_struct).in_dll(host, "parts")
             C.memmove(s.parts, src, n * C.sizeof(abi.part_struct))

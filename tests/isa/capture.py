@@ -5,7 +5,9 @@ before and after, for replay against the gfx950 machine code (replay.py).
 
     python tests/isa/capture.py <case> <outdir> [steps] [fp64|mixed] [implicit]
 
-Cases are the small parity cases of tests/test_gpu_parity.py; the optional
+Cases are the small parity cases of tests/test_gpu_parity.py (and
+`precursor` / `precursor_x`, the cases of tests/test_gpu_precursor.py:
+planes pushed, cuda_dom_BC, cuda_dom_BC_star, cuda_yank_turb_planes); the optional
 fourth argument selects the Poisson solver form and `implicit` the
 Crank-Nicolson (Helmholtz) predictor."""
 import os
@@ -29,6 +31,38 @@ def main():
     from test_gpu_parity import CASES
     import dataclasses
     from bluebottle_amd import runtime, sim
+    if case_name in ("precursor", "precursor_x"):
+        # the PRECURSOR ghost fill and the plane extraction on random
+        # fields and planes (tests/test_gpu_precursor.py's cases)
+        import ctypes as C
+        import numpy as np
+        from test_gpu_precursor import CASES as PCASES
+        from test_oracle_precursor import configs18
+        case = PCASES["all" if case_name == "precursor" else "inflow_x"]()
+        s = sim.Simulation(case)
+        rng = np.random.default_rng(5)
+        for nm in ("u", "v", "w", "u_star", "v_star", "w_star", "p"):
+            a = s.array(nm)
+            a[:] = rng.standard_normal(a.size)
+        for nm in s.PLANES:
+            a = s.array(nm)
+            a[:] = rng.standard_normal(a.size)
+        s.to_device()
+        cfg = configs18(case)
+        s.hip.cuda_dom_turb_planes_push(cfg)
+        s.hip.cuda_dom_BC()
+        s.hip.cuda_dom_BC_star()
+        d = s.dom
+        pos = (C.c_double * 9)(d.xs + 0.37 * d.xl, 0, 0, 0,
+                               d.ys + 0.61 * d.yl, 0, 0, 0,
+                               d.zs + 0.43 * d.zl)
+        vel = (C.c_double * 18)(*[0.01 * (q + 1) for q in range(18)])
+        s.hip.cuda_yank_turb_planes(cfg, pos, vel)
+        s.pull()
+        s.close()
+        print("captured", len([f for f in os.listdir(out) if f.endswith(".json")]),
+              "launches of", case_name, flush=True)
+        return
     if case_name == "copy":
         # the HBM copy reference kernel: 64 KiB, two launches
         _, hip = runtime.load()
@@ -49,6 +83,9 @@ def main():
     s = sim.Simulation(case)
     s.to_device()
     s.set_solver(mode)
+    if os.environ.get("BB_CAPTURE_INNER_SWEEPS"):
+        # the mixed form's inner solve in two sweeps per iteration
+        s.hip.bb_set_inner_sweeps(int(os.environ["BB_CAPTURE_INNER_SWEEPS"]))
     s.prepare()
     if os.environ.get("BB_CAPTURE_HELPERS") == "1":
         # the stand-alone entry points the step does not call
