@@ -58,6 +58,13 @@ void out_restart(void);
 void in_restart(void);
 /* src/recorder.c:1955-2001, appending to ROOT_DIR/record/<name> */
 void recorder_lamb(char *name, int iter);
+/* record.config (src/recorder.c:26-371) into the rec_* globals; 0 or -1 */
+int bb_read_record_config(const char *path);
+/* the reference's names: ROOT_DIR/input/record.config (exits on error),
+ * and the creation of ROOT_DIR/record/<name> */
+void recorder_read_config(void);
+void recorder_bicgstab_init(char *name);
+void recorder_lamb_init(char *name);
 
 /* ---- ParaView output (bb_vtk.c, src/vtk.c) into directory `dir` ---- */
 int bb_vtk_init(const char *dir);             /* empty out.pvd */

@@ -33,6 +33,17 @@ real rec_restart_ttime_out;
 real rec_prec_ttime_out;
 real bc_plane_pos[9];
 
+/* output settings of record.config (src/bluebottle.c:139-169) */
+int rec_flow_field_stepnum_out, rec_particle_stepnum_out;
+int rec_prec_stepnum_out, rec_prec_flow_field_stepnum_out;
+real rec_flow_field_dt, rec_paraview_dt, rec_particle_dt, rec_restart_dt;
+real rec_prec_dt, rec_prec_flow_field_dt, rec_prec_flow_field_ttime_out;
+int rec_flow_field_vel, rec_flow_field_p, rec_flow_field_phase;
+int rec_prec_flow_field_vel, rec_prec_flow_field_p, rec_prec_flow_field_phase;
+int rec_restart_stop;
+int rec_particle_pos, rec_particle_a, rec_particle_vel, rec_particle_omega;
+int rec_particle_force, rec_particle_moment;
+
 /* the directory holding input/ and record/ (ROOT_DIR of the reference) */
 char bb_root_dir[FILE_NAME_SIZE] = ".";
 
@@ -488,4 +499,132 @@ void recorder_lamb(char *name, int iter)
     }
   }
   fclose(rec);
+}
+
+/* ---------------------------------------------------------------------
+ * record.config (src/recorder.c:26-371): output intervals, one "TYPE dt"
+ * line each (dt < 0 turns the output off); FLOW_FIELD and PARTICLE take
+ * option lines up to the next blank line, every other type consumes the
+ * line after it.  As in the reference, only "velocity" and "pressure"
+ * are accepted after FLOW_FIELD and only "position" after PARTICLE, and
+ * each type line is tested against the types in table order with the
+ * buffer as the earlier types' handling left it.  Departure: an option
+ * list that runs to the end of the file ends there (the reference keeps
+ * re-reading its last line).
+ * ------------------------------------------------------------------- */
+static const char *REC_TYPES[7] = {"FLOW_FIELD", "PARAVIEW", "PARTICLE",
+  "RESTART", "PRECURSOR_VTK", "RESTART_STOP", "PRECURSOR"};
+
+/* option lines after FLOW_FIELD / PARTICLE; -1 on an unknown option */
+static int rec_options(FILE *f, char *buf, int len, const char *type,
+  const char *const *opts, int *const *flags, int nopt)
+{
+  if(fgets(buf, len, f) == NULL) return 0;
+  while(strcmp("\n", buf) != 0) {
+    size_t ln = strlen(buf);
+    if(ln > 0 && buf[ln - 1] == '\n') buf[ln - 1] = '\0';
+    int found = 0;
+    for(int j = 0; j < nopt; j++)
+      if(strcmp(opts[j], buf) == 0) { *flags[j] = 1; found = 1; }
+    if(!found) {
+      fprintf(stderr, "Unrecognized record.config %s option %s\n", type, buf);
+      return -1;
+    }
+    if(fgets(buf, len, f) == NULL) break;
+  }
+  return 0;
+}
+
+int bb_read_record_config(const char *path)
+{
+  rec_flow_field_dt = -1; rec_flow_field_vel = 0; rec_flow_field_p = 0;
+  rec_flow_field_phase = 0;
+  rec_paraview_dt = -1;
+  rec_particle_dt = -1; rec_particle_pos = 0; rec_particle_a = 0;
+  rec_particle_vel = 0; rec_particle_omega = 0; rec_particle_force = 0;
+  rec_particle_moment = 0;
+  rec_restart_dt = -1; rec_prec_dt = -1; rec_restart_stop = 0;
+  rec_prec_flow_field_dt = -1; rec_prec_flow_field_vel = 0;
+  rec_prec_flow_field_p = 0; rec_prec_flow_field_phase = 0;
+  FILE *f = fopen(path, "r");
+  if(f == NULL) {
+    fprintf(stderr, "Could not open file %s\n", path);
+    return -1;
+  }
+  static const char *const flow_opts[2] = {"velocity", "pressure"};
+  int *const flow_flags[2] = {&rec_flow_field_vel, &rec_flow_field_p};
+  static const char *const part_opts[1] = {"position"};
+  int *const part_flags[1] = {&rec_particle_pos};
+  char buf[CHAR_BUF_SIZE] = "", word[CHAR_BUF_SIZE] = "";
+  while(fgets(buf, CHAR_BUF_SIZE, f) != NULL) {
+    int found = 0;
+    for(int i = 0; i < 7; i++) {
+      real dtout = 0;
+      sscanf(buf, "%s %lf\n", word, &dtout);
+      if(strcmp(REC_TYPES[i], word) != 0) continue;
+      found = 1;
+      int err = 0;
+      switch(i) {
+        case 0: rec_flow_field_dt = dtout;
+          err = rec_options(f, buf, CHAR_BUF_SIZE, word, flow_opts, flow_flags, 2);
+          break;
+        case 2: rec_particle_dt = dtout;
+          err = rec_options(f, buf, CHAR_BUF_SIZE, word, part_opts, part_flags, 1);
+          break;
+        default: {
+          real *dst[7] = {NULL, &rec_paraview_dt, NULL, &rec_restart_dt,
+            &rec_prec_dt, NULL, &rec_prec_flow_field_dt};
+          if(i == 5) rec_restart_stop = (int) dtout;
+          else *dst[i] = dtout;
+          if(fgets(buf, CHAR_BUF_SIZE, f) == NULL) buf[0] = '\0';
+          break;
+        }
+      }
+      if(err) { fclose(f); return -1; }
+    }
+    if(!found) {
+      fprintf(stderr, "Unrecognized record.config type %s\n", word);
+      fclose(f);
+      return -1;
+    }
+  }
+  fclose(f);
+  return 0;
+}
+
+void recorder_read_config(void)
+{
+  char path[2 * FILE_NAME_SIZE];
+  snprintf(path, sizeof(path), "%s/input/record.config", bb_root_dir);
+  if(bb_read_record_config(path)) exit(EXIT_FAILURE);
+}
+
+/* src/recorder.c:1891-1925: create record/<name> (the solver log with its
+ * column header, the Lamb log empty) */
+static FILE *record_file(const char *name)
+{
+  char path[2 * FILE_NAME_SIZE + 16];
+  snprintf(path, sizeof(path), "%s/record/%s", bb_root_dir, name);
+  FILE *rec = fopen(path, "w");
+  if(rec == NULL) {
+    fprintf(stderr, "Could not open file %s\n", name);
+    exit(EXIT_FAILURE);
+  }
+  return rec;
+}
+
+void recorder_bicgstab_init(char *name)
+{
+  FILE *rec = record_file(name);
+  fprintf(rec, "%-12s", "stepnum");
+  fprintf(rec, "%-15s", "ttime");
+  fprintf(rec, "%-15s", "dt");
+  fprintf(rec, "%-8s", "niter");
+  fprintf(rec, "%-15s", "resid");
+  fclose(rec);
+}
+
+void recorder_lamb_init(char *name)
+{
+  fclose(record_file(name));
 }

@@ -74,6 +74,9 @@ HOST_FUNCS = {
     "bb_out_restart": (C.c_int, [C.c_char_p]),
     "bb_in_restart": (C.c_int, [C.c_char_p]),
     "recorder_lamb": (None, [C.c_char_p, C.c_int]),
+    "bb_read_record_config": (C.c_int, [C.c_char_p]),
+    "recorder_bicgstab_init": (None, [C.c_char_p]),
+    "recorder_lamb_init": (None, [C.c_char_p]),
     "bb_vtk_init": (C.c_int, [C.c_char_p]),
     "bb_vtk_grid": (C.c_int, [C.c_char_p, C.c_int]),
     "bb_vtk_parts": (C.c_int, [C.c_char_p, C.c_int]),

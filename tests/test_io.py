@@ -275,3 +275,64 @@ def test_lamb_log(tmp_path):
     assert float(rows[1].split()[2]) == 1.5
     assert text[9] == "particle 1"
     s.close()
+
+
+# ---------------------------------------------------------------------------
+# record.config and the recorder files
+# ---------------------------------------------------------------------------
+def _rec(name, kind="d"):
+    return runtime.get_scalar(name, kind)
+
+
+def test_reference_record_config():
+    """sim/input/record.config of the reference: every output interval."""
+    import os
+    host, _ = runtime.load()
+    path = os.path.join(os.path.dirname(__file__), "golden", "ref_input",
+                        "record.config")
+    assert host.bb_read_record_config(path.encode()) == 0
+    assert _rec("rec_flow_field_dt") == -1
+    assert _rec("rec_particle_dt") == -0.01
+    assert _rec("rec_paraview_dt") == 0.01
+    assert _rec("rec_prec_flow_field_dt") == -0.01     # PRECURSOR
+    assert _rec("rec_prec_dt") == -0.01                # PRECURSOR_VTK
+    assert _rec("rec_restart_dt") == 1.0
+    assert _rec("rec_restart_stop", "i") == 1
+    assert _rec("rec_flow_field_vel", "i") == 0
+
+
+def test_record_config_options(tmp_path):
+    host, _ = runtime.load()
+    p = tmp_path / "record.config"
+    p.write_text("FLOW_FIELD 0.5\nvelocity\npressure\n\nPARTICLE 0.25\n"
+                 "position\n\nRESTART 2\n\nPARAVIEW -1\n")
+    assert host.bb_read_record_config(str(p).encode()) == 0
+    assert (_rec("rec_flow_field_dt"), _rec("rec_flow_field_vel", "i"),
+            _rec("rec_flow_field_p", "i")) == (0.5, 1, 1)
+    assert (_rec("rec_particle_dt"), _rec("rec_particle_pos", "i")) == (0.25, 1)
+    assert (_rec("rec_restart_dt"), _rec("rec_paraview_dt")) == (2.0, -1.0)
+    # untouched outputs take their defaults (off)
+    assert _rec("rec_prec_dt") == -1 and _rec("rec_restart_stop", "i") == 0
+    # the reference accepts only "position" after PARTICLE
+    p.write_text("PARTICLE 0.25\nvelocity\n\n")
+    assert host.bb_read_record_config(str(p).encode()) == -1
+    p.write_text("NOSUCH 1\n")
+    assert host.bb_read_record_config(str(p).encode()) == -1
+    # an option list running to the end of the file
+    p.write_text("FLOW_FIELD 1\nvelocity")
+    assert host.bb_read_record_config(str(p).encode()) == 0
+    assert _rec("rec_flow_field_vel", "i") == 1
+
+
+def test_recorder_init_files(tmp_path):
+    host, _ = runtime.load()
+    buf = C.c_char * 256
+    buf.in_dll(host, "bb_root_dir").value = str(tmp_path).encode()
+    (tmp_path / "record").mkdir()
+    host.recorder_bicgstab_init(b"solver_expd.rec")
+    host.recorder_lamb_init(b"lamb.rec")
+    head = (tmp_path / "record" / "solver_expd.rec").read_text()
+    assert head == "%-12s%-15s%-15s%-8s%-15s" % ("stepnum", "ttime", "dt",
+                                                 "niter", "resid")
+    assert (tmp_path / "record" / "lamb.rec").read_text() == ""
+    buf.in_dll(host, "bb_root_dir").value = b"."
