@@ -72,12 +72,19 @@ int bb_vtk_grid(const char *dir, int step);   /* out_N.pvtr, out_N_0.vtr */
 int bb_vtk_parts(const char *dir, int step);  /* out_N.vtp */
 int bb_vtk_nodes(const char *dir, int step);  /* out_nodes_N.vtp */
 int bb_vtk_out(const char *dir, int step);    /* all of them + out.pvd */
+/* the precursor domain's files (src/vtk.c:324-611) */
+int bb_vtk_turb_init(const char *dir);        /* empty out_turb.pvd */
+int bb_vtk_turb_grid(const char *dir, int step); /* out_turb_N.pvtr, _N_0.vtr */
+int bb_vtk_turb_out(const char *dir, int step);  /* both + out_turb.pvd */
 /* the reference's names, on ROOT_DIR/output; they exit on error */
 void init_VTK(void);
 void out_VTK(void);
 void dom_out_VTK(void);
 void part_out_VTK(void);
 void quadnodes_out_VTK(void);
+void init_VTK_turb(void);
+void out_VTK_turb(void);
+void dom_out_VTK_turb(void);
 
 #ifdef __cplusplus
 }

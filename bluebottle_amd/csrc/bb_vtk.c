@@ -12,6 +12,9 @@
  *                       forces (total F adds the buoyant weight)
  *   out_nodes_N.vtp     the 26 Lebedev nodes of every point of out_N.vtp
  *                       at r = rs, with their interference codes
+ *   out_turb.pvd, out_turb_N.pvtr, out_turb_N_0.vtr
+ *                       the precursor domain's output (src/vtk.c:324-611):
+ *                       one DataSet per output, cell data p and vel only
  * The images follow the reference's order: for each particle, the single
  * face images (x low, x high, y low, y high, z low, z high), the edge
  * images of the xy, zy and xz pairs, the corner images, then the particle
@@ -32,7 +35,7 @@
 #include "../../include/bluebottle.h"
 #include "bb_host.h"
 
-extern int rec_paraview_stepnum_out;
+extern int rec_paraview_stepnum_out, rec_prec_stepnum_out;
 
 #define VTK_PI 3.14159265358979323846
 
@@ -108,12 +111,12 @@ static FILE *open_out(const char *dir, const char *fmt, int step)
 }
 
 /* ---------------------------------------------------------------------
- * out.pvd
+ * Collections: out.pvd (main domain), out_turb.pvd (precursor domain)
  * ------------------------------------------------------------------- */
-int bb_vtk_init(const char *dir)
+static int pvd_init(const char *dir, const char *name)
 {
   char path[2 * FILE_NAME_SIZE];
-  snprintf(path, sizeof(path), "%s/out.pvd", dir);
+  snprintf(path, sizeof(path), "%s/%s", dir, name);
   FILE *f = fopen(path, "w");
   if(f == NULL) {
     fprintf(stderr, "Could not open file %s\n", path);
@@ -125,24 +128,28 @@ int bb_vtk_init(const char *dir)
   return 0;
 }
 
-static int pvd_append(const char *dir, int step)
+int bb_vtk_init(const char *dir) { return pvd_init(dir, "out.pvd"); }
+int bb_vtk_turb_init(const char *dir) { return pvd_init(dir, "out_turb.pvd"); }
+
+/* append one DataSet per entry of kind[] (printf patterns of the step) */
+static int pvd_append(const char *dir, const char *name, int step,
+  const char *const *kind, int nkind)
 {
   char path[2 * FILE_NAME_SIZE];
-  snprintf(path, sizeof(path), "%s/out.pvd", dir);
+  snprintf(path, sizeof(path), "%s/%s", dir, name);
   FILE *f = fopen(path, "r+");
   if(f == NULL) {
-    if(bb_vtk_init(dir)) return -1;
+    if(pvd_init(dir, name)) return -1;
     f = fopen(path, "r+");
     if(f == NULL) return -1;
   }
   /* overwrite the 24-byte footer "</Collection>\n</VTKFile>" */
   fseek(f, -24, SEEK_END);
-  const char *kind[3] = {"out_%d.pvtr", "out_%d.vtp", "out_nodes_%d.vtp"};
-  for(int part = 0; part < 3; part++) {
-    char name[64];
-    snprintf(name, sizeof(name), kind[part], step);
+  for(int part = 0; part < nkind; part++) {
+    char file[64];
+    snprintf(file, sizeof(file), kind[part], step);
     fprintf(f, "<DataSet timestep=\"%e\" part=\"%d\" file=\"%s\"/>\n", ttime,
-      part, name);
+      part, file);
   }
   fprintf(f, "</Collection>\n</VTKFile>");
   fclose(f);
@@ -150,53 +157,56 @@ static int pvd_append(const char *dir, int step)
 }
 
 /* ---------------------------------------------------------------------
- * Grid: out_N.pvtr + out_N_0.vtr (one subdomain)
+ * Grid: PREFIX_N.pvtr + PREFIX_N_0.vtr (one subdomain).  The main domain
+ * (src/vtk.c:26-322) writes p, phase and vel and rounds the piece extent
+ * of the header; the precursor domain (src/vtk.c:324-611) writes p and vel
+ * and truncates it.  The piece file truncates in both.
  * ------------------------------------------------------------------- */
-int bb_vtk_grid(const char *dir, int step)
+static int grid_files(const char *dir, const char *prefix, int step,
+  int with_phase, int round_header)
 {
   const grid_info *gc = &Dom.Gcc;
   const dom_struct *sd = &dom[0];
-  FILE *f = open_out(dir, "out_%d.pvtr", step);
+  char fmt[64];
+  snprintf(fmt, sizeof(fmt), "%s_%%d.pvtr", prefix);
+  FILE *f = open_out(dir, fmt, step);
   if(f == NULL) return -1;
-  /* the piece extent of the header rounds, the one of the piece file
-   * truncates (both as the reference computes them) */
-  int xl = (int) round((sd->xe - sd->xs) / Dom.dx);
-  int xs0 = (int) round((sd->xs - Dom.xs) / Dom.dx);
-  int yl = (int) round((sd->ye - sd->ys) / Dom.dy);
-  int ys0 = (int) round((sd->ys - Dom.ys) / Dom.dy);
-  int zl = (int) round((sd->ze - sd->zs) / Dom.dz);
-  int zs0 = (int) round((sd->zs - Dom.zs) / Dom.dz);
+  const real ext[6] = {(sd->xe - sd->xs) / Dom.dx, (sd->xs - Dom.xs) / Dom.dx,
+    (sd->ye - sd->ys) / Dom.dy, (sd->ys - Dom.ys) / Dom.dy,
+    (sd->ze - sd->zs) / Dom.dz, (sd->zs - Dom.zs) / Dom.dz};
+  int e[6];
+  for(int q = 0; q < 6; q++)
+    e[q] = round_header ? (int) round(ext[q]) : (int) ext[q];
+  const char *scalars = with_phase ? "p phase" : "p";
   fprintf(f, "<VTKFile type=\"PRectilinearGrid\">\n");
   fprintf(f, "<PRectilinearGrid WholeExtent=\"0 %d 0 %d 0 %d\" "
     "GhostLevel=\"0\">\n", Dom.xn, Dom.yn, Dom.zn);
-  fprintf(f, "<PCellData Scalars=\"p phase\" Vectors=\"vel\">\n");
+  fprintf(f, "<PCellData Scalars=\"%s\" Vectors=\"vel\">\n", scalars);
   fprintf(f, "<PDataArray type=\"Float32\" Name=\"p\"/>\n");
-  fprintf(f, "<PDataArray type=\"Int32\" Name=\"phase\"/>\n");
+  if(with_phase) fprintf(f, "<PDataArray type=\"Int32\" Name=\"phase\"/>\n");
   fprintf(f, "<PDataArray type=\"Float32\" Name=\"vel\" "
     "NumberOfComponents=\"3\"/>\n");
   fprintf(f, "</PCellData>\n<PCoordinates>\n");
   for(int a = 0; a < 3; a++)
     fprintf(f, "<PDataArray type=\"Float32\" Name=\"%c\"/>\n", "xyz"[a]);
   fprintf(f, "</PCoordinates>\n");
-  fprintf(f, "<Piece Extent=\"%d %d %d %d %d %d\" Source=\"out_%d_0.vtr\"/>\n",
-    xs0, xs0 + xl, ys0, ys0 + yl, zs0, zs0 + zl, step);
+  fprintf(f, "<Piece Extent=\"%d %d %d %d %d %d\" Source=\"%s_%d_0.vtr\"/>\n",
+    e[1], e[1] + e[0], e[3], e[3] + e[2], e[5], e[5] + e[4], prefix, step);
   fprintf(f, "</PRectilinearGrid>\n</VTKFile>\n");
   fclose(f);
 
-  xl = (int) ((sd->xe - sd->xs) / Dom.dx);
-  xs0 = (int) ((sd->xs - Dom.xs) / Dom.dx);
-  yl = (int) ((sd->ye - sd->ys) / Dom.dy);
-  ys0 = (int) ((sd->ys - Dom.ys) / Dom.dy);
-  zl = (int) ((sd->ze - sd->zs) / Dom.dz);
-  zs0 = (int) ((sd->zs - Dom.zs) / Dom.dz);
-  f = open_out(dir, "out_%d_0.vtr", step);
+  const int xl = (int) ext[0], xs0 = (int) ext[1];
+  const int yl = (int) ext[2], ys0 = (int) ext[3];
+  const int zl = (int) ext[4], zs0 = (int) ext[5];
+  snprintf(fmt, sizeof(fmt), "%s_%%d_0.vtr", prefix);
+  f = open_out(dir, fmt, step);
   if(f == NULL) return -1;
   fprintf(f, "<VTKFile type=\"RectilinearGrid\">\n");
   fprintf(f, "<RectilinearGrid WholeExtent=\"0 %d 0 %d 0 %d\" "
     "GhostLevel=\"0\">\n", Dom.xn, Dom.yn, Dom.zn);
   fprintf(f, "<Piece Extent=\"%d %d %d %d %d %d\">\n", xs0, xs0 + xl, ys0,
     ys0 + yl, zs0, zs0 + zl);
-  fprintf(f, "<CellData Scalars=\"p phase\" Vectors=\"vel\">\n");
+  fprintf(f, "<CellData Scalars=\"%s\" Vectors=\"vel\">\n", scalars);
   const int k0 = zs0 + gc->ks, k1 = zs0 + zl + gc->ks;
   const int j0 = ys0 + gc->js, j1 = ys0 + yl + gc->js;
   const int i0 = xs0 + gc->is, i1 = xs0 + xl + gc->is;
@@ -206,12 +216,14 @@ int bb_vtk_grid(const char *dir, int step)
       for(int i = i0; i < i1; i++)
         fprintf(f, "%e ", p[i + j * gc->s1b + k * gc->s2b]);
   fprintf(f, "\n</DataArray>\n");
-  fprintf(f, "<DataArray type=\"Int32\" Name=\"phase\">\n");
-  for(int k = k0; k < k1; k++)
-    for(int j = j0; j < j1; j++)
-      for(int i = i0; i < i1; i++)
-        fprintf(f, "%d ", phase[i + j * gc->s1b + k * gc->s2b]);
-  fprintf(f, "\n</DataArray>\n");
+  if(with_phase) {
+    fprintf(f, "<DataArray type=\"Int32\" Name=\"phase\">\n");
+    for(int k = k0; k < k1; k++)
+      for(int j = j0; j < j1; j++)
+        for(int i = i0; i < i1; i++)
+          fprintf(f, "%d ", phase[i + j * gc->s1b + k * gc->s2b]);
+    fprintf(f, "\n</DataArray>\n");
+  }
   /* face velocities averaged to the cell centre, computed as written */
   fprintf(f, "<DataArray type=\"Float32\" Name=\"vel\" NumberOfComponents=\"3\""
     " format=\"ascii\">\n");
@@ -237,6 +249,16 @@ int bb_vtk_grid(const char *dir, int step)
   fprintf(f, "</Coordinates>\n</Piece>\n</RectilinearGrid>\n</VTKFile>\n");
   fclose(f);
   return 0;
+}
+
+int bb_vtk_grid(const char *dir, int step)
+{
+  return grid_files(dir, "out", step, 1, 1);
+}
+
+int bb_vtk_turb_grid(const char *dir, int step)
+{
+  return grid_files(dir, "out_turb", step, 0, 0);
 }
 
 /* ---------------------------------------------------------------------
@@ -403,10 +425,20 @@ int bb_vtk_nodes(const char *dir, int step)
 /* one ParaView output: the collection entry and the three files */
 int bb_vtk_out(const char *dir, int step)
 {
-  if(pvd_append(dir, step)) return -1;
+  static const char *const kind[3] = {"out_%d.pvtr", "out_%d.vtp",
+    "out_nodes_%d.vtp"};
+  if(pvd_append(dir, "out.pvd", step, kind, 3)) return -1;
   if(bb_vtk_grid(dir, step)) return -1;
   if(bb_vtk_parts(dir, step)) return -1;
   return bb_vtk_nodes(dir, step);
+}
+
+/* one precursor-domain output: its collection entry and the grid files */
+int bb_vtk_turb_out(const char *dir, int step)
+{
+  static const char *const kind[1] = {"out_turb_%d.pvtr"};
+  if(pvd_append(dir, "out_turb.pvd", step, kind, 1)) return -1;
+  return bb_vtk_turb_grid(dir, step);
 }
 
 /* ---- the reference's names, on ROOT_DIR/output, exiting on error ---- */
@@ -448,4 +480,25 @@ void quadnodes_out_VTK(void)
   char d[2 * FILE_NAME_SIZE];
   output_dir(d, sizeof(d));
   if(bb_vtk_nodes(d, rec_paraview_stepnum_out)) exit(EXIT_FAILURE);
+}
+
+void init_VTK_turb(void)
+{
+  char d[2 * FILE_NAME_SIZE];
+  output_dir(d, sizeof(d));
+  if(bb_vtk_turb_init(d)) exit(EXIT_FAILURE);
+}
+
+void out_VTK_turb(void)
+{
+  char d[2 * FILE_NAME_SIZE];
+  output_dir(d, sizeof(d));
+  if(bb_vtk_turb_out(d, rec_prec_stepnum_out)) exit(EXIT_FAILURE);
+}
+
+void dom_out_VTK_turb(void)
+{
+  char d[2 * FILE_NAME_SIZE];
+  output_dir(d, sizeof(d));
+  if(bb_vtk_turb_grid(d, rec_prec_stepnum_out)) exit(EXIT_FAILURE);
 }

@@ -82,6 +82,9 @@ HOST_FUNCS = {
     "bb_vtk_parts": (C.c_int, [C.c_char_p, C.c_int]),
     "bb_vtk_nodes": (C.c_int, [C.c_char_p, C.c_int]),
     "bb_vtk_out": (C.c_int, [C.c_char_p, C.c_int]),
+    "bb_vtk_turb_init": (C.c_int, [C.c_char_p]),
+    "bb_vtk_turb_grid": (C.c_int, [C.c_char_p, C.c_int]),
+    "bb_vtk_turb_out": (C.c_int, [C.c_char_p, C.c_int]),
 }
 
 EXTRA_HIP = {

@@ -203,3 +203,55 @@ def test_collection(written):
         ("7.500000e-01", "0", "out_1.pvtr"), ("7.500000e-01", "1", "out_1.vtp"),
         ("7.500000e-01", "2", "out_nodes_1.vtp")]
     assert text.endswith("</Collection>\n</VTKFile>")
+
+
+def test_precursor_domain_files(tmp_path):
+    """out_turb_N.pvtr/.vtr carry p and vel only, the header's piece extent
+    truncates (0.9 / (0.9 / 7) = 6.999... gives 6 cells in x, where the
+    main header rounds to 7), the piece file truncates as the main one
+    does, and out_turb.pvd gains one DataSet per output."""
+    case = sim.Case(n=(7, 5, 4), hi=(0.9, 1.0, 0.8), nu=0.01,
+                    init="taylor_green", noise=1e-3, seed=2)
+    s = sim.Simulation(case)
+    try:
+        rng = np.random.default_rng(3)
+        for nm in ("p", "u", "v", "w"):
+            a = s.array(nm)
+            a[:] = rng.standard_normal(a.size)
+        d = str(tmp_path).encode()
+        runtime.set_scalar("ttime", 0.25)
+        assert s.host.bb_vtk_turb_init(d) == 0
+        assert s.host.bb_vtk_turb_out(d, 3) == 0
+        runtime.set_scalar("ttime", 0.5)
+        assert s.host.bb_vtk_turb_out(d, 4) == 0
+        assert s.host.bb_vtk_grid(d, 4) == 0
+        root = ET.parse(tmp_path / "out_turb_4_0.vtr").getroot()
+        cd = root.find("RectilinearGrid/Piece/CellData")
+        assert cd.get("Scalars") == "p"
+        assert root.find("RectilinearGrid/Piece").get("Extent") == "0 6 0 5 0 4"
+        A = _arrays(root)
+        assert sorted(A) == ["p", "vel", "x", "y", "z"]
+        gc = s.dom.Gcc
+        p3 = s.array("p").reshape(gc.knb, gc.jnb, gc.inb)[1:-1, 1:-1, 1:7]
+        assert [float(x) for x in A["p"]] == [float("%e" % x)
+                                              for x in p3.ravel()]
+        u = s.field3("u")[1:-1, 1:-1, :]
+        uc = 0.5 * (u[:, :, 1:7] + u[:, :, 2:8])
+        vel = np.array([float(x) for x in A["vel"]]).reshape(-1, 3)
+        assert np.array_equal(vel[:, 0], [float("%e" % x) for x in uc.ravel()])
+        assert len(A["x"]) == 7
+        hdr = ET.parse(tmp_path / "out_turb_4.pvtr").getroot()
+        names = [x.get("Name") for x in hdr.iter("PDataArray")]
+        assert names == ["p", "vel", "x", "y", "z"]
+        pc = hdr.find("PRectilinearGrid/Piece")
+        assert pc.get("Extent") == "0 6 0 5 0 4"
+        assert pc.get("Source") == "out_turb_4_0.vtr"
+        main = ET.parse(tmp_path / "out_4.pvtr").getroot()
+        assert main.find("PRectilinearGrid/Piece").get("Extent") == "0 7 0 5 0 4"
+        ds = ET.parse(tmp_path / "out_turb.pvd").getroot().findall(
+            "Collection/DataSet")
+        assert [(x.get("timestep"), x.get("part"), x.get("file"))
+                for x in ds] == [("2.500000e-01", "0", "out_turb_3.pvtr"),
+                                 ("5.000000e-01", "0", "out_turb_4.pvtr")]
+    finally:
+        s.close()
