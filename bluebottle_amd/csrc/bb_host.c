@@ -14,6 +14,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #include "../../include/bluebottle.h"
 #include "bb_host.h"
 
@@ -61,6 +62,7 @@ int init_cond;
 gradP_struct gradP;
 long int cpumem, gpumem;
 real pid_int, pid_back, Kp, Ki, Kd;
+real turbA, turbl;
 /* ---- src/domain.c:27 ---- */
 int nsubdom;
 /* ---- src/particle.c:29-75 ---- */
@@ -222,6 +224,7 @@ int bb_host_domain(real xs, real xe, int xn, real ys, real ye, int yn,
   nparts = 0;
   parts = NULL;
   coeff_stride = 0;
+  turbl = 0.;
   return 0;
 }
 
@@ -448,4 +451,205 @@ int bb_host_step(void)
     return -1;
   }
   return iter;
+}
+
+/* ---------------------------------------------------------------------
+ * The precursor domain (the turbulence rank of src/bluebottle.c:850-1156)
+ * ------------------------------------------------------------------- */
+
+/* p[idx] where idx may lie past the cell array: the v correction below
+ * indexes p with v's strides, which reach past p when zn > yn + 2 */
+static real p_at(long idx)
+{
+  return idx < (long) Dom.Gcc.s3b ? p[idx] : 0.;
+}
+
+/* src/domain.c:2856-3093 after the grids and arrays exist (bb_host_domain):
+ * uniform random velocities of amplitude 3 turbA l (l the mean side
+ * length, the integral scale turbl), the reference's one-sweep divergence
+ * correction and mean removal, zeroed inflow planes, and the first dt
+ * from the viscous limit alone.  The generator is seeded with `seed`
+ * (the reference uses the wall clock).  As there, the correction of v
+ * indexes the cell divergence with v's own strides; values that would lie
+ * past the cell array (zn > yn + 2) are taken as 0, where the reference
+ * reads beyond it. */
+int bb_host_init_turb(unsigned long long seed)
+{
+  const grid_info *X = &Dom.Gfx, *Y = &Dom.Gfy, *Z = &Dom.Gfz, *G = &Dom.Gcc;
+  memset(p0, 0, sizeof(real) * G->s3b);
+  memset(p, 0, sizeof(real) * G->s3b);
+  memset(phi, 0, sizeof(real) * G->s3b);
+  real *face[3][8] = {
+    {u, diff0_u, diff_u, u0, conv0_u, conv_u, f_x, u_star},
+    {v, diff0_v, diff_v, v0, conv0_v, conv_v, f_y, v_star},
+    {w, diff0_w, diff_w, w0, conv0_w, conv_w, f_z, w_star}};
+  const int n3[3] = {X->s3b, Y->s3b, Z->s3b};
+  for(int c = 0; c < 3; c++)
+    for(int a = 0; a < 8; a++) memset(face[c][a], 0, sizeof(real) * n3[c]);
+
+  rng_init(seed);
+  turbl = (Dom.xl + Dom.yl + Dom.zl) / 3.;
+  const real urms = 3 * turbA * turbl;
+  for(int c = 0; c < 3; c++) {
+    real *x = face[c][0], *x0 = face[c][3];
+    for(int i = 0; i < n3[c]; i++) {
+      const real tmp = (rng_dbl() - 0.5) * urms;
+      x[i] = tmp;
+      x0[i] = tmp;
+    }
+  }
+
+  /* the divergence over every cell, ghosts included, over the cell count */
+  const real vol = Dom.xn * Dom.yn * Dom.zn;
+  for(int k = G->ksb; k < G->keb; k++)
+    for(int j = G->jsb; j < G->jeb; j++)
+      for(int i = G->isb; i < G->ieb; i++) {
+        const int W = i + j * X->s1b + k * X->s2b;
+        const int S = i + j * Y->s1b + k * Y->s2b;
+        const int B = i + j * Z->s1b + k * Z->s2b;
+        const int C = i + j * G->s1b + k * G->s2b;
+        p[C] = (u[W + 1] - u[W]) / Dom.dx + (v[S + Y->s1b] - v[S]) / Dom.dy
+          + (w[B + Z->s2b] - w[B]) / Dom.dz;
+        p[C] = p[C] / vol;
+      }
+  for(int k = X->ks; k < X->ke; k++)
+    for(int j = X->js; j < X->je; j++)
+      for(int i = X->is; i < X->ie; i++) {
+        const int C = i + j * X->s1b + k * X->s2b;
+        const int E = i + j * G->s1b + k * G->s2b;
+        u[C] = u[C] - 0.5 * (p[E - 1] + p[E]);
+        u0[C] = u0[C] - 0.5 * (p[E - 1] + p[E]);
+      }
+  for(int k = Y->ks; k < Y->ke; k++)
+    for(int j = Y->js; j < Y->je; j++)
+      for(int i = Y->is; i < Y->ie; i++) {
+        const long C = i + (long) j * Y->s1b + (long) k * Y->s2b;
+        const real ps = p_at(C - Y->s1b), pn = p_at(C);
+        v[C] = v[C] - 0.5 * (ps + pn);
+        v0[C] = v0[C] - 0.5 * (ps + pn);
+      }
+  for(int k = Z->ks; k < Z->ke; k++)
+    for(int j = Z->js; j < Z->je; j++)
+      for(int i = Z->is; i < Z->ie; i++) {
+        const int C = i + j * Z->s1b + k * Z->s2b;
+        w[C] = w[C] - 0.5 * (p[C - Z->s2b] + p[C]);
+        w0[C] = w0[C] - 0.5 * (p[C - Z->s2b] + p[C]);
+      }
+
+  /* means over the faces less the last one on each component's axis */
+  real umean = 0., vmean = 0., wmean = 0.;
+  for(int k = X->ks; k < X->ke; k++)
+    for(int j = X->js; j < X->je; j++)
+      for(int i = X->is; i < X->ie - 1; i++) umean += u[i + j * X->s1b + k * X->s2b];
+  for(int k = Y->ks; k < Y->ke; k++)
+    for(int j = Y->js; j < Y->je - 1; j++)
+      for(int i = Y->is; i < Y->ie; i++) vmean += v[i + j * Y->s1b + k * Y->s2b];
+  for(int k = Z->ks; k < Z->ke - 1; k++)
+    for(int j = Z->js; j < Z->je; j++)
+      for(int i = Z->is; i < Z->ie; i++) wmean += w[i + j * Z->s1b + k * Z->s2b];
+  umean /= vol;
+  vmean /= vol;
+  wmean /= vol;
+  const real mean[3] = {umean, vmean, wmean};
+  for(int c = 0; c < 3; c++) {
+    real *x = face[c][0], *x0 = face[c][3];
+    for(int i = 0; i < n3[c]; i++) {
+      x[i] = x[i] - mean[c];
+      x0[i] = x0[i] - mean[c];
+    }
+  }
+
+  real *planes[15] = {u_WE, u_SN_S, u_SN_N, u_BT_B, u_BT_T, v_WE_W, v_WE_E,
+    v_SN, v_BT_B, v_BT_T, w_WE_W, w_WE_E, w_SN_S, w_SN_N, w_BT};
+  const size_t psz[15] = {(size_t) X->jnb * X->knb, (size_t) X->inb * X->knb,
+    (size_t) X->inb * X->knb, (size_t) X->inb * X->jnb,
+    (size_t) X->inb * X->jnb, (size_t) Y->jnb * Y->knb,
+    (size_t) Y->jnb * Y->knb, (size_t) Y->inb * Y->knb,
+    (size_t) Y->inb * Y->jnb, (size_t) Y->inb * Y->jnb,
+    (size_t) Z->jnb * Z->knb, (size_t) Z->jnb * Z->knb,
+    (size_t) Z->inb * Z->knb, (size_t) Z->inb * Z->knb,
+    (size_t) Z->inb * Z->jnb};
+  for(int q = 0; q < 15; q++) memset(planes[q], 0, sizeof(real) * psz[q]);
+
+  real dx_min = Dom.dx;
+  if(Dom.dy < dx_min) dx_min = Dom.dy;
+  if(Dom.dz < dx_min) dx_min = Dom.dz;
+  dt = 2. * nu / (dx_min * dx_min);
+  dt = CFL / dt;
+  dt0 = -1.;
+  stepnum = 0;
+  bb_reset_output_clocks();
+  return 0;
+}
+
+/* src/domain.c:2349: the grids and arrays of the Dom that turb.config
+ * gave, then the initial field seeded from the wall clock */
+int domain_init_turb(void)
+{
+  if(nsubdom > dev_end - dev_start + 1) return EXIT_FAILURE;
+  bb_host_domain(Dom.xs, Dom.xe, Dom.xn, Dom.ys, Dom.ye, Dom.yn, Dom.zs,
+    Dom.ze, Dom.zn);
+  bb_host_init_turb((unsigned long long) time(NULL));
+  return EXIT_SUCCESS;
+}
+
+/* src/bluebottle.c:945-1016 after the pushes: cages (of no particles),
+ * the first dt from the random field, the BCs */
+int bb_host_prepare_turb(void)
+{
+  cuda_build_cages();
+  cuda_part_pull();
+  dt = cuda_find_dt();
+  cuda_dom_BC();
+  return 0;
+}
+
+/* src/bluebottle.c:945-1016 with restart requested: the restart file
+ * replaces the host state and is pushed before the first dt and BCs */
+int bb_host_restart_turb(const char *path)
+{
+  cuda_build_cages();
+  cuda_part_pull();
+  if(bb_in_restart_turb(path) != 0) return -1;
+  cuda_dom_push();
+  cuda_part_push();
+  dt = cuda_find_dt();
+  cuda_dom_BC();
+  return 0;
+}
+
+/* src/bluebottle.c:1036-1081: one precursor time step, without the
+ * exchange with the main domain and the output; returns 0, or -1 when
+ * the new dt fails the driver's blow-up test */
+int bb_host_step_turb(void)
+{
+  ttime += dt;
+  stepnum++;
+  cuda_compute_forcing(&pid_int, &pid_back, Kp, Ki, Kd);
+  cuda_compute_turb_forcing();
+  compute_vel_BC();
+  if(!bb_is_implicit()) cuda_U_star_2();
+  else {
+    cuda_ustar_helmholtz(1);
+    cuda_vstar_helmholtz(1);
+    cuda_wstar_helmholtz(1);
+  }
+  cuda_dom_BC_star();
+  cuda_solvability();
+  cuda_dom_BC_star();
+  cuda_PP_bicgstab(1);
+  cuda_dom_BC_phi();
+  cuda_project();
+  cuda_dom_BC();
+  cuda_update_p();
+  cuda_dom_BC_p();
+  cuda_store_u();
+  dt0 = dt;
+  dt = cuda_find_dt();
+  if(!(dt >= 1.e-10 && dt <= 1.e10)) {
+    printf("The solution has diverged.  Ending simulation.\n");
+    fflush(stdout);
+    return -1;
+  }
+  return 0;
 }

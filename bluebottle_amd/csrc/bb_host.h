@@ -40,6 +40,33 @@ int bb_host_restart(const char *path);
  * when the solution has diverged (the driver's dt test). */
 int bb_host_step(void);
 
+/* ---- the precursor domain (src/bluebottle.c:850-1156) ---- */
+/* the random initial field of src/domain.c:2856-3093 on the arrays that
+ * bb_host_domain made, seeded with `seed`; sets turbl and the first dt */
+int bb_host_init_turb(unsigned long long seed);
+/* src/domain.c:2349 (grids, arrays, the field seeded from the clock);
+ * EXIT_SUCCESS or EXIT_FAILURE */
+int domain_init_turb(void);
+/* src/bluebottle.c:945-1016 after the pushes */
+int bb_host_prepare_turb(void);
+/* the same with the precursor's restart file in place of its set-up */
+int bb_host_restart_turb(const char *path);
+/* src/bluebottle.c:1036-1081: one step; 0, or -1 when it diverged */
+int bb_host_step_turb(void);
+/* zero the output step counters and clocks */
+void bb_reset_output_clocks(void);
+
+/* ---- random numbers (bb_rng.c; src/rng.h) ---- */
+extern unsigned long long int rng_j, rng_v;
+void rng_init(unsigned long long int seed);
+unsigned long long int rng_uint64(void);
+unsigned int rng_uint32(void);
+long long int rng_int64(void);
+int rng_int32(void);
+float rng_flt(void);
+double rng_dbl(void);
+double gaussrand(void);
+
 /* ---- files in the reference's formats (bb_io.c) ---- */
 /* directory holding input/ and record/ (the reference's ROOT_DIR, ".") */
 extern char bb_root_dir[FILE_NAME_SIZE];
@@ -51,8 +78,16 @@ int bb_read_part_config(const char *path);
 /* the raw restart dump of src/domain.c:3223-3413 (host arrays) */
 int bb_out_restart(const char *path);
 int bb_in_restart(const char *path);
+/* turb.config (src/domain.c:732-1287), the precursor domain's file */
+int bb_read_turb_config(const char *path);
+/* the precursor's dump (src/domain.c:3415-3546) */
+int bb_out_restart_turb(const char *path);
+int bb_in_restart_turb(const char *path);
 /* the reference's names, on ROOT_DIR/input/...; they exit on error */
 void domain_read_input(void);
+void turb_read_input(void);
+void out_restart_turb(void);
+void in_restart_turb(void);
 void parts_read_input(int turb);
 void out_restart(void);
 void in_restart(void);

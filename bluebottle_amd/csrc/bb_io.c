@@ -44,6 +44,21 @@ int rec_restart_stop;
 int rec_particle_pos, rec_particle_a, rec_particle_vel, rec_particle_omega;
 int rec_particle_force, rec_particle_moment;
 
+/* the output counters and clocks domain_init and domain_init_turb zero
+ * (src/domain.c:2334-2343, 3081-3090) */
+void bb_reset_output_clocks(void)
+{
+  rec_flow_field_stepnum_out = 0;
+  rec_paraview_stepnum_out = 0;
+  rec_particle_stepnum_out = 0;
+  rec_prec_stepnum_out = 0;
+  rec_flow_field_ttime_out = 0;
+  rec_paraview_ttime_out = 0;
+  rec_particle_ttime_out = 0;
+  rec_restart_ttime_out = 0;
+  rec_prec_ttime_out = 0;
+}
+
 /* the directory holding input/ and record/ (ROOT_DIR of the reference) */
 char bb_root_dir[FILE_NAME_SIZE] = ".";
 
@@ -100,8 +115,9 @@ static real *bc_ds(int q)
   return d[q];
 }
 
-/* one "bc.<c><F> TYPE [values]" line */
-static int read_bc_line(FILE *f, char c, int q)
+/* one "bc.<c><F> TYPE [values]" line; the precursor domain's own file
+ * knows no PRECURSOR faces (src/domain.c:898-1219) */
+static int read_bc_line(FILE *f, char c, int q, int allow_precursor)
 {
   char pat[32], buf[CHAR_BUF_SIZE] = "";
   snprintf(pat, sizeof(pat), "bc.%c%c %%255s", c, FACES[q]);
@@ -121,7 +137,8 @@ static int read_bc_line(FILE *f, char c, int q)
     real *dm = bc_val(c, q, 0), *d = bc_val(c, q, 1), *da = bc_val(c, q, 2);
     if(strcmp(buf, "PERIODIC") == 0) {
       *type = PERIODIC;
-    } else if(strcmp(buf, "DIRICHLET") == 0 || strcmp(buf, "PRECURSOR") == 0) {
+    } else if(strcmp(buf, "DIRICHLET") == 0 ||
+              (allow_precursor && strcmp(buf, "PRECURSOR") == 0)) {
       *type = buf[0] == 'D' ? DIRICHLET : PRECURSOR;
       if(fscanf(f, "%lf %lf", dm, da) != 2) return -1;
       *d = 0;
@@ -167,26 +184,24 @@ static void read_int(FILE *f, const char *pat, int *x)
   if(fscanf(f, pat, x) < 0) {}
 }
 
-int bb_read_flow_config(const char *path)
+/* DOMAIN, GPU DOMAIN DECOMPOSITION and PHYSICAL PARAMETERS, shared by
+ * flow.config and turb.config (src/domain.c:52-104, 754-802); 0 or -1
+ * (the file is closed on error) */
+static int read_domain_head(FILE *f, const char *what)
 {
-  FILE *f = fopen(path, "r");
-  if(f == NULL) {
-    fprintf(stderr, "Could not open file %s\n", path);
-    return -1;
-  }
   cpumem = 0;
   gpumem = 0;
   skip(f, "DOMAIN\n");
   if(fscanf(f, "(Xs, Xe, Xn) %lf %lf %d\n", &Dom.xs, &Dom.xe, &Dom.xn) != 3 ||
      fscanf(f, "(Ys, Ye, Yn) %lf %lf %d\n", &Dom.ys, &Dom.ye, &Dom.yn) != 3 ||
      fscanf(f, "(Zs, Ze, Zn) %lf %lf %d\n", &Dom.zs, &Dom.ze, &Dom.zn) != 3)
-    return read_error(f, "flow.config");
+    return read_error(f, what);
   skip(f, "\n");
   skip(f, "GPU DOMAIN DECOMPOSITION\n");
   if(fscanf(f, "DEV RANGE %d %d\n", &dev_start, &dev_end) != 2 ||
      fscanf(f, "n %d\n", &nsubdom) != 1 || nsubdom != 1) {
-    fprintf(stderr, "flow.config: one subdomain (one GPU) expected\n");
-    return read_error(f, "flow.config");
+    fprintf(stderr, "%s: one subdomain (one GPU) expected\n", what);
+    return read_error(f, what);
   }
   /* the subdomain record (src/domain.c:68-91) */
   dom_struct sub;
@@ -196,13 +211,58 @@ int bb_read_flow_config(const char *path)
      fscanf(f, "(Zs, Ze, Zn) %lf %lf %d\n", &sub.zs, &sub.ze, &sub.zn) != 3 ||
      fscanf(f, "E %d W %d N %d S %d T %d B %d\n", &sub.E, &sub.W, &sub.N,
        &sub.S, &sub.T, &sub.B) != 6)
-    return read_error(f, "flow.config");
+    return read_error(f, what);
   skip(f, "\n");
   skip(f, "PHYSICAL PARAMETERS\n");
   read_real(f, "rho_f %lf\n", &rho_f);
   read_real(f, "nu %lf\n", &nu);
   mu = nu * rho_f;
   skip(f, "\n");
+  return 0;
+}
+
+/* the four BC groups (PRESSURE, X-, Y-, Z-VELOCITY); 0 or -1 */
+static int read_bc_groups(FILE *f, const char *what, int allow_precursor)
+{
+  const char *groups[4] = {"PRESSURE\n", "X-VELOCITY\n", "Y-VELOCITY\n",
+    "Z-VELOCITY\n"};
+  const char comps[4] = {'p', 'u', 'v', 'w'};
+  for(int g = 0; g < 4; g++) {
+    skip(f, groups[g]);
+    for(int q = 0; q < 6; q++)
+      if(read_bc_line(f, comps[g], q, allow_precursor))
+        return read_error(f, what);
+    if(g == 0 && (bc.dsE < 0 || bc.dsW < 0 || bc.dsN < 0 || bc.dsS < 0 ||
+                  bc.dsT < 0 || bc.dsB < 0)) {
+      fprintf(stderr, "%s read error -- screen offsets must be >= 0\n", what);
+      fclose(f);
+      return -1;
+    }
+  }
+  return 0;
+}
+
+static int read_out_plane(FILE *f, const char *what)
+{
+  const char *names[7] = {"WEST", "EAST", "SOUTH", "NORTH", "BOTTOM", "TOP",
+    "HOMOGENEOUS"};
+  const int vals[7] = {WEST, EAST, SOUTH, NORTH, BOTTOM, TOP, HOMOGENEOUS};
+  skip(f, "SOLVABILITY ENFORCEMENT PLANE\n");
+  if(read_keyword(f, "out_plane %255s", names, vals, 7, &out_plane))
+    return read_error(f, what);
+  skip(f, "\n");
+  skip(f, "\n");
+  return 0;
+}
+
+int bb_read_flow_config(const char *path)
+{
+  FILE *f = fopen(path, "r");
+  if(f == NULL) {
+    fprintf(stderr, "Could not open file %s\n", path);
+    return -1;
+  }
+  if(read_domain_head(f, "flow.config")) return -1;
   skip(f, "SIMULATION PARAMETERS\n");
   read_real(f, "duration %lf\n", &duration);
   read_real(f, "CFL %lf\n", &CFL);
@@ -215,21 +275,7 @@ int bb_read_flow_config(const char *path)
   skip(f, "\n");
   skip(f, "BOUNDARY CONDITIONS\n");
   read_real(f, "vel_tDelay %lf\n", &vel_tDelay);
-  const char *groups[4] = {"PRESSURE\n", "X-VELOCITY\n", "Y-VELOCITY\n",
-    "Z-VELOCITY\n"};
-  const char comps[4] = {'p', 'u', 'v', 'w'};
-  for(int g = 0; g < 4; g++) {
-    skip(f, groups[g]);
-    for(int q = 0; q < 6; q++)
-      if(read_bc_line(f, comps[g], q)) return read_error(f, "flow.config");
-    if(g == 0 && (bc.dsE < 0 || bc.dsW < 0 || bc.dsN < 0 || bc.dsS < 0 ||
-                  bc.dsT < 0 || bc.dsB < 0)) {
-      fprintf(stderr, "flow.config read error -- screen offsets must be "
-        ">= 0\n");
-      fclose(f);
-      return -1;
-    }
-  }
+  if(read_bc_groups(f, "flow.config", 1)) return -1;
   skip(f, "\n");
   skip(f, "INITIAL CONDITION\n");
   {
@@ -240,16 +286,7 @@ int bb_read_flow_config(const char *path)
   }
   skip(f, "\n");
   skip(f, "\n");
-  skip(f, "SOLVABILITY ENFORCEMENT PLANE\n");
-  {
-    const char *names[7] = {"WEST", "EAST", "SOUTH", "NORTH", "BOTTOM", "TOP",
-      "HOMOGENEOUS"};
-    const int vals[7] = {WEST, EAST, SOUTH, NORTH, BOTTOM, TOP, HOMOGENEOUS};
-    if(read_keyword(f, "out_plane %255s", names, vals, 7, &out_plane))
-      return read_error(f, "flow.config");
-  }
-  skip(f, "\n");
-  skip(f, "\n");
+  if(read_out_plane(f, "flow.config")) return -1;
   skip(f, "SIMULATION DRIVING CONDITIONS\n");
   read_real(f, "p_tDelay %lf\n", &p_tDelay);
   if(fscanf(f, "gradP.x %lf %lf\n", &gradP.xm, &gradP.xa) < 0) {}
@@ -266,6 +303,7 @@ int bb_read_flow_config(const char *path)
   read_real(f, "Kp %lf\n", &Kp);
   read_real(f, "Ki %lf\n", &Ki);
   read_real(f, "Kd %lf\n", &Kd);
+  turbA = 0.;
   fclose(f);
   return 0;
 }
@@ -276,6 +314,54 @@ void domain_read_input(void)
   char path[2 * FILE_NAME_SIZE];
   snprintf(path, sizeof(path), "%s/input/flow.config", bb_root_dir);
   if(bb_read_flow_config(path)) exit(EXIT_FAILURE);
+}
+
+/* ---------------------------------------------------------------------
+ * turb.config: the precursor domain's file (src/domain.c:732-1287).  The
+ * flow.config grammar without the Lamb parameters, vel_tDelay, the
+ * PRECURSOR face type, the initial condition and the PID gains; its
+ * driving section holds only turbA.  The settings the precursor does not
+ * use are zeroed as there.
+ * ------------------------------------------------------------------- */
+int bb_read_turb_config(const char *path)
+{
+  FILE *f = fopen(path, "r");
+  if(f == NULL) {
+    fprintf(stderr, "Could not open file %s\n", path);
+    return -1;
+  }
+  if(read_domain_head(f, "turb.config")) return -1;
+  skip(f, "SIMULATION PARAMETERS\n");
+  read_real(f, "duration %lf\n", &duration);
+  read_real(f, "CFL %lf\n", &CFL);
+  read_int(f, "pp_max_iter %d\n", &pp_max_iter);
+  read_real(f, "pp_residual %lf\n", &pp_residual);
+  lamb_max_iter = 0;
+  lamb_residual = 0.;
+  lamb_relax = 0.;
+  lamb_cut = 0.;
+  skip(f, "\n");
+  skip(f, "BOUNDARY CONDITIONS\n");
+  vel_tDelay = 0;
+  if(read_bc_groups(f, "turb.config", 0)) return -1;
+  skip(f, "\n");
+  if(read_out_plane(f, "turb.config")) return -1;
+  skip(f, "SIMULATION DRIVING CONDITIONS\n");
+  read_real(f, "turbA %lf\n", &turbA);
+  p_tDelay = 0;
+  memset(&gradP, 0, sizeof(gradP));
+  g_tDelay = 0;
+  memset(&g, 0, sizeof(g));
+  fclose(f);
+  return 0;
+}
+
+/* src/domain.c:732 (ROOT_DIR/input/turb.config; exits on error) */
+void turb_read_input(void)
+{
+  char path[2 * FILE_NAME_SIZE];
+  snprintf(path, sizeof(path), "%s/input/turb.config", bb_root_dir);
+  if(bb_read_turb_config(path)) exit(EXIT_FAILURE);
 }
 
 /* ---------------------------------------------------------------------
@@ -404,17 +490,80 @@ static int restart_layout(chunk_t *c, int for_read)
   return n;
 }
 
-int bb_out_restart(const char *path)
+/* the precursor domain's dump (src/domain.c:3415-3546): the same head
+ * and field arrays, the plane positions, then its two output clocks; no
+ * particles, coefficients or controller state */
+static int restart_turb_layout(chunk_t *c)
+{
+  const int diff0 = !bb_is_implicit();
+  const size_t gx = Dom.Gfx.s3b, gy = Dom.Gfy.s3b, gz = Dom.Gfz.s3b;
+  const size_t gc = Dom.Gcc.s3b;
+  int n = 0;
+  c[n++] = (chunk_t) {&ttime, sizeof(real), 1};
+  c[n++] = (chunk_t) {&dt0, sizeof(real), 1};
+  c[n++] = (chunk_t) {&dt, sizeof(real), 1};
+  c[n++] = (chunk_t) {&stepnum, sizeof(int), 1};
+  c[n++] = (chunk_t) {&rec_paraview_stepnum_out, sizeof(int), 1};
+  n = face_chunks(c, n, u, u0, diff0_u, conv0_u, diff_u, conv_u, u_star, gx,
+    diff0);
+  n = face_chunks(c, n, v, v0, diff0_v, conv0_v, diff_v, conv_v, v_star, gy,
+    diff0);
+  n = face_chunks(c, n, w, w0, diff0_w, conv0_w, diff_w, conv_w, w_star, gz,
+    diff0);
+  c[n++] = (chunk_t) {p, sizeof(real), gc};
+  c[n++] = (chunk_t) {phi, sizeof(real), gc};
+  c[n++] = (chunk_t) {p0, sizeof(real), gc};
+  c[n++] = (chunk_t) {phase, sizeof(int), gc};
+  c[n++] = (chunk_t) {phase_shell, sizeof(int), gc};
+  c[n++] = (chunk_t) {flag_u, sizeof(int), gx};
+  c[n++] = (chunk_t) {flag_v, sizeof(int), gy};
+  c[n++] = (chunk_t) {flag_w, sizeof(int), gz};
+  c[n++] = (chunk_t) {bc_plane_pos, sizeof(real), 9};
+  c[n++] = (chunk_t) {&rec_prec_ttime_out, sizeof(real), 1};
+  c[n++] = (chunk_t) {&rec_prec_flow_field_ttime_out, sizeof(real), 1};
+  return n;
+}
+
+static int write_chunks(const char *path, const chunk_t *c, int n)
 {
   FILE *f = fopen(path, "w");
   if(f == NULL) {
-    fprintf(stderr, "Could not open file restart.input.\n");
+    fprintf(stderr, "Could not open file %s\n", path);
     return -1;
   }
-  chunk_t c[96];
-  int n = restart_layout(c, 0);
   for(int i = 0; i < n; i++)
     if(c[i].count && fwrite(c[i].p, c[i].size, c[i].count, f) != c[i].count) {
+      fclose(f);
+      return -1;
+    }
+  fclose(f);
+  return 0;
+}
+
+int bb_out_restart(const char *path)
+{
+  chunk_t c[96];
+  return write_chunks(path, c, restart_layout(c, 0));
+}
+
+int bb_out_restart_turb(const char *path)
+{
+  chunk_t c[64];
+  return write_chunks(path, c, restart_turb_layout(c));
+}
+
+int bb_in_restart_turb(const char *path)
+{
+  FILE *f = fopen(path, "r");
+  if(f == NULL) {
+    fprintf(stderr, "Could not open file %s\n", path);
+    return -1;
+  }
+  chunk_t c[64];
+  const int n = restart_turb_layout(c);
+  for(int i = 0; i < n; i++)
+    if(c[i].count && fread(c[i].p, c[i].size, c[i].count, f) != c[i].count) {
+      fprintf(stderr, "restart: file ends early\n");
       fclose(f);
       return -1;
     }
@@ -466,6 +615,21 @@ void in_restart(void)
   char path[2 * FILE_NAME_SIZE];
   snprintf(path, sizeof(path), "%s/input/restart.config", bb_root_dir);
   if(bb_in_restart(path)) exit(EXIT_FAILURE);
+}
+
+/* src/domain.c:3415 and 3480 (ROOT_DIR/input/restart_turb.config) */
+void out_restart_turb(void)
+{
+  char path[2 * FILE_NAME_SIZE];
+  snprintf(path, sizeof(path), "%s/input/restart_turb.config", bb_root_dir);
+  if(bb_out_restart_turb(path)) exit(EXIT_FAILURE);
+}
+
+void in_restart_turb(void)
+{
+  char path[2 * FILE_NAME_SIZE];
+  snprintf(path, sizeof(path), "%s/input/restart_turb.config", bb_root_dir);
+  if(bb_in_restart_turb(path)) exit(EXIT_FAILURE);
 }
 
 /* ---------------------------------------------------------------------

@@ -1,8 +1,7 @@
 /*
  * bb_outside.hip -- entry points the reference driver links that lie
- * outside this library's path: the precursor domain's turbulence forcing
- * (src/precursor.c, src/bluebottle.c:956-1044), the -DTEST checks
- * of src/cuda_testing.cu, and a few helpers no caller uses.  They exist so
+ * outside this library's path: the -DTEST checks of src/cuda_testing.cu
+ * and a few helpers no caller uses.  They exist so
  * that the unchanged src/bluebottle.c links against this library; each one
  * prints which path it belongs to and exits, as a missing feature should
  * rather than run wrong.
@@ -19,12 +18,7 @@ static void outside(const char *name, const char *what)
   exit(EXIT_FAILURE);
 }
 
-#define PRECURSOR_PATH "precursor inflow"
 #define TEST_PATH "-DTEST check"
-
-/* src/bluebottle.h:2877 (the precursor domain's turbulence forcing; the
- * planes themselves are in bb_precursor.hip) */
-extern "C" void cuda_compute_turb_forcing(void) { outside(__func__, PRECURSOR_PATH); }
 
 /* src/bluebottle.h:2616, 2628, 2753, 2766, 2780, 2792 */
 extern "C" void cuda_U_star_test_exp(void) { outside(__func__, TEST_PATH); }
@@ -34,12 +28,8 @@ extern "C" void cuda_BC_test(void) { outside(__func__, TEST_PATH); }
 extern "C" void cuda_quad_interp_test(void) { outside(__func__, TEST_PATH); }
 extern "C" void cuda_lamb_test(void) { outside(__func__, TEST_PATH); }
 
-/* src/bluebottle.h:2704, 2717, 2943, 3341, 3355-3391: declared, not called
- * on the path (cuda_div_U's one call is commented out in bluebottle.c:700) */
+/* src/bluebottle.h:2704, 2717, 2943: declared, not called on the path
+ * (cuda_div_U's one call is commented out in bluebottle.c:700) */
 extern "C" void cuda_PP_bicgstab_jacobi(void) { outside(__func__, "unused solver variant"); }
 extern "C" void cuda_div_U(void) { outside(__func__, "divergence diagnostic"); }
 extern "C" void cuda_collisions(void) { outside(__func__, "unused collision variant"); }
-extern "C" real cuda_compute_energy(void) { outside(__func__, "energy diagnostic"); return 0.; }
-extern "C" void cuda_colocate_Gfx(real *, real *, dom_struct *) { outside(__func__, "output colocation"); }
-extern "C" void cuda_colocate_Gfy(real *, real *, dom_struct *) { outside(__func__, "output colocation"); }
-extern "C" void cuda_colocate_Gfz(real *, real *, dom_struct *) { outside(__func__, "output colocation"); }

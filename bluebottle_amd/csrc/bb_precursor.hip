@@ -266,3 +266,210 @@ extern "C" void cuda_yank_turb_planes(int *bc_flow_configs, real *pos,
     yank(_w[0], d->Gfz, 2, 1, _w_BT[0], NULL, pos[8], vel[17], d->dz, d->zs,
       1. / d->dx, 0);
 }
+
+/* ---------------------------------------------------------------------
+ * The precursor domain's linear forcing (src/cuda_bluebottle.cu:3817-4120,
+ * src/bluebottle_kernel.cu:2054-2131,3849-3906).
+ *
+ * The reference colocates u, v and w into three cell arrays (twice: once
+ * for the energy, once for the means), multiplies them into a fourth,
+ * reduces each with Thrust on the host's clock, then adds the field and
+ * the constant in six launches.  Here one pass over the cells forms the
+ * three cell-centred values in registers and leaves per-block partials of
+ * sum(uc), sum(vc), sum(wc) and sum(uc^2 + vc^2 + wc^2); one block sums
+ * them in a fixed order and computes k, the means, the gain turbA k0 / k
+ * and the three constants on the device; one launch over the three face
+ * arrays adds both terms.  Nothing returns to the host unless the entry
+ * point returns k.  Each term is rounded where the reference rounds it
+ * (no fused multiply-add), so the result differs from a serial sum only
+ * by the order of the sums.
+ * ------------------------------------------------------------------- */
+#define TURB_BLOCKS 512
+
+/* slots of g_bb.ws.red used here */
+#define TURB_OUT 40   /* k, umean, vmean, wmean, gain, cu, cv, cw */
+
+__global__ void __launch_bounds__(256)
+k_turb_sums(BBGeom G, const double *__restrict__ u,
+  const double *__restrict__ v, const double *__restrict__ w,
+  double *__restrict__ part)
+{
+#pragma clang fp contract(off)
+  __shared__ double sh[4];
+  const long n = (long) G.xn * G.yn * G.zn;
+  double su = 0., sv = 0., sw = 0., se = 0.;
+  for(long q = (long) blockIdx.x * 256 + threadIdx.x; q < n;
+      q += (long) gridDim.x * 256) {
+    const int i = (int) (q % G.xn) + 1;
+    const int j = (int) ((q / G.xn) % G.yn) + 1;
+    const int k = (int) (q / ((long) G.xn * G.yn)) + 1;
+    const int cx = i + j * G.us1 + k * G.us2;
+    const int cy = i + j * G.vs1 + k * G.vs2;
+    const int cz = i + j * G.ws1 + k * G.ws2;
+    const double uc = 0.5 * (u[cx] + u[cx + 1]);
+    const double vc = 0.5 * (v[cy] + v[cy + G.vs1]);
+    const double wc = 0.5 * (w[cz] + w[cz + G.ws2]);
+    su += uc;
+    sv += vc;
+    sw += wc;
+    se += uc * uc + vc * vc + wc * wc;
+  }
+  const double a = bb_block_sum(su, sh);
+  const double b = bb_block_sum(sv, sh);
+  const double c = bb_block_sum(sw, sh);
+  const double e = bb_block_sum(se, sh);
+  if(threadIdx.x == 0) {
+    part[4 * blockIdx.x + 0] = a;
+    part[4 * blockIdx.x + 1] = b;
+    part[4 * blockIdx.x + 2] = c;
+    part[4 * blockIdx.x + 3] = e;
+  }
+}
+
+/* one block: out[] = k, the three means, the gain and the three constants,
+ * each in the reference's operation order */
+__global__ void __launch_bounds__(256)
+k_turb_final(int nblk, double ncell, double turbA_, double turbl_,
+  const double *__restrict__ part, double *__restrict__ out)
+{
+#pragma clang fp contract(off)
+  __shared__ double sh[4];
+  double a = 0., b = 0., c = 0., e = 0.;
+  for(int q = threadIdx.x; q < nblk; q += 256) {
+    a += part[4 * q];
+    b += part[4 * q + 1];
+    c += part[4 * q + 2];
+    e += part[4 * q + 3];
+  }
+  a = bb_block_sum(a, sh);
+  b = bb_block_sum(b, sh);
+  c = bb_block_sum(c, sh);
+  e = bb_block_sum(e, sh);
+  if(threadIdx.x != 0) return;
+  const double k = e / ncell * 0.5;
+  const double um = a / ncell, vm = b / ncell, wm = c / ncell;
+  const double k0 = 13.5 * turbA_ * turbA_ * 0.0361 * turbl_ * turbl_;
+  out[0] = k;
+  out[1] = um;
+  out[2] = vm;
+  out[3] = wm;
+  out[4] = turbA_ * k0 / k;
+  out[5] = -turbA_ * k0 / k * um;
+  out[6] = -turbA_ * k0 / k * vm;
+  out[7] = -turbA_ * k0 / k * wm;
+}
+
+/* f += gain * vel, then f += constant, over every value of the three
+ * ghosted face arrays (one flat index over u's, then v's, then w's) */
+__global__ void __launch_bounds__(256)
+k_turb_apply(int nu_, int nv_, int nw_, const double *__restrict__ u,
+  const double *__restrict__ v, const double *__restrict__ w,
+  double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
+  const double *__restrict__ out)
+{
+#pragma clang fp contract(off)
+  const long n = (long) nu_ + nv_ + nw_;
+  const double gain = out[4];
+  for(long q = (long) blockIdx.x * 256 + threadIdx.x; q < n;
+      q += (long) gridDim.x * 256) {
+    const double *x;
+    double *f;
+    double c;
+    long r = q;
+    if(r < nu_) { x = u; f = fx; c = out[5]; }
+    else if((r -= nu_) < nv_) { x = v; f = fy; c = out[6]; }
+    else { r -= nv_; x = w; f = fz; c = out[7]; }
+    double t = f[r] + gain * x[r];
+    f[r] = t + c;
+  }
+}
+
+/* the sums and the constants, left in g_bb.ws.red + TURB_OUT */
+static void turb_reduce(void)
+{
+  bb_ensure_workspace();
+  BBGeom G = bb_geom();
+  k_turb_sums<<<TURB_BLOCKS, 256, 0, g_bb.stream>>>(G, _u[0], _v[0], _w[0],
+    g_bb.ws.partials);
+  BB_CHECK_LAUNCH();
+  const double ncell = (double) G.xn * G.yn * G.zn;
+  k_turb_final<<<1, 256, 0, g_bb.stream>>>(TURB_BLOCKS, ncell, turbA, turbl,
+    g_bb.ws.partials, g_bb.ws.red + TURB_OUT);
+  BB_CHECK_LAUNCH();
+}
+
+extern "C" real cuda_compute_energy(void)
+{
+  bb_prof_begin(BB_K_MISC);
+  turb_reduce();
+  bb_prof_end(BB_K_MISC);
+  BB_CHECK(hipMemcpyAsync(g_bb.ws.red_host, g_bb.ws.red + TURB_OUT,
+    sizeof(double), hipMemcpyDeviceToHost, g_bb.stream));
+  BB_CHECK(hipStreamSynchronize(g_bb.stream));
+  return g_bb.ws.red_host[0];
+}
+
+extern "C" void cuda_compute_turb_forcing(void)
+{
+  bb_prof_begin(BB_K_MISC);
+  turb_reduce();
+  BBGeom G = bb_geom();
+  const long n = (long) G.ucount + G.vcount + G.wcount;
+  long nb = (n + 255) / 256;
+  if(nb > 4096) nb = 4096;
+  k_turb_apply<<<(int) nb, 256, 0, g_bb.stream>>>(G.ucount, G.vcount,
+    G.wcount, _u[0], _v[0], _w[0], _f_x[0], _f_y[0], _f_z[0],
+    g_bb.ws.red + TURB_OUT);
+  BB_CHECK_LAUNCH();
+  bb_prof_end(BB_K_MISC);
+}
+
+/* colocate_Gf{x,y,z}: cell c (interior index, no ghosts) gets the mean of
+ * the two faces of component `comp` that bound it; the geometry is read
+ * from the device copy of the subdomain the caller passes */
+__global__ void __launch_bounds__(256)
+k_colocate(int comp, const dom_struct *__restrict__ d,
+  const double *__restrict__ f, double *__restrict__ out)
+{
+  const grid_info &G = comp == 0 ? d->Gfx : comp == 1 ? d->Gfy : d->Gfz;
+  const int nx = d->Gcc.in, ny = d->Gcc.jn, nz = d->Gcc.kn;
+  const long n = (long) nx * ny * nz;
+  const int step = comp == 0 ? 1 : comp == 1 ? G.s1b : G.s2b;
+  for(long q = (long) blockIdx.x * 256 + threadIdx.x; q < n;
+      q += (long) gridDim.x * 256) {
+    const int i = (int) (q % nx), j = (int) ((q / nx) % ny);
+    const int k = (int) (q / ((long) nx * ny));
+    const int C = (i + DOM_BUF) + (j + DOM_BUF) * G.s1b + (k + DOM_BUF) * G.s2b;
+    out[i + j * d->Gcc.s1 + k * d->Gcc.s2] = 0.5 * (f[C] + f[C + step]);
+  }
+}
+
+static void colocate(int comp, real *f, real *out, dom_struct *d_dom)
+{
+  bb_ensure_context();
+  if(f == NULL || out == NULL || d_dom == NULL) {
+    fprintf(stderr, "cuda_colocate: null device pointer\n");
+    exit(EXIT_FAILURE);
+  }
+  const long n = (long) dom[0].Gcc.in * dom[0].Gcc.jn * dom[0].Gcc.kn;
+  long nb = (n + 255) / 256;
+  if(nb > 4096) nb = 4096;
+  if(nb < 1) nb = 1;
+  k_colocate<<<(int) nb, 256, 0, g_bb.stream>>>(comp, d_dom, f, out);
+  BB_CHECK_LAUNCH();
+}
+
+extern "C" void cuda_colocate_Gfx(real *_u_, real *_u_co, dom_struct *_dom_)
+{
+  colocate(0, _u_, _u_co, _dom_);
+}
+
+extern "C" void cuda_colocate_Gfy(real *_v_, real *_v_co, dom_struct *_dom_)
+{
+  colocate(1, _v_, _v_co, _dom_);
+}
+
+extern "C" void cuda_colocate_Gfz(real *_w_, real *_w_co, dom_struct *_dom_)
+{
+  colocate(2, _w_, _w_co, _dom_);
+}

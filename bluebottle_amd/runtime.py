@@ -85,6 +85,17 @@ HOST_FUNCS = {
     "bb_vtk_turb_init": (C.c_int, [C.c_char_p]),
     "bb_vtk_turb_grid": (C.c_int, [C.c_char_p, C.c_int]),
     "bb_vtk_turb_out": (C.c_int, [C.c_char_p, C.c_int]),
+    "bb_read_turb_config": (C.c_int, [C.c_char_p]),
+    "bb_out_restart_turb": (C.c_int, [C.c_char_p]),
+    "bb_in_restart_turb": (C.c_int, [C.c_char_p]),
+    "bb_host_init_turb": (C.c_int, [C.c_ulonglong]),
+    "bb_host_prepare_turb": (C.c_int, []),
+    "bb_host_restart_turb": (C.c_int, [C.c_char_p]),
+    "bb_host_step_turb": (C.c_int, []),
+    "rng_init": (None, [C.c_ulonglong]),
+    "rng_uint64": (C.c_ulonglong, []),
+    "rng_dbl": (C.c_double, []),
+    "gaussrand": (C.c_double, []),
 }
 
 EXTRA_HIP = {

@@ -73,6 +73,12 @@ class Case:
     interaction_length_ratio: float = 1.0
     # Crank-Nicolson diffusion (the reference's -DIMPLICIT build)
     implicit: bool = False
+    # the precursor domain (the reference's turbulence rank): the random
+    # initial field of amplitude 3 turbA l drawn from `seed` instead of
+    # `init`, the linear forcing turbA (k0/k)(u - <u>) every step, no
+    # particles and no Lamb iteration
+    precursor: bool = False
+    turbA: float = 0.0
     particles: list = field(default_factory=list)
 
 
@@ -139,6 +145,7 @@ class Simulation:
         rs("g_tDelay", c.g_tDelay)
         rs("vel_tDelay", c.vel_tDelay)
         rs("interactionLengthRatio", c.interaction_length_ratio)
+        rs("turbA", c.turbA)
         for k in ("Kp", "Ki", "Kd", "pid_int", "pid_back", "ttime"):
             rs(k, 0.0)
         b = runtime.bc()
@@ -161,7 +168,11 @@ class Simulation:
         self.host.bb_host_domain(c.lo[0], c.hi[0], nx, c.lo[1], c.hi[1], ny,
                                  c.lo[2], c.hi[2], nz)
         self.dom = runtime.dom()
-        if c.init == "taylor_green":
+        if c.precursor:
+            if c.particles:
+                raise ValueError("the precursor domain carries no particles")
+            self.host.bb_host_init_turb(int(c.seed))
+        elif c.init == "taylor_green":
             self._taylor_green()
         elif c.init == "shear":
             self._shear()
@@ -312,18 +323,26 @@ class Simulation:
         self.device = True
 
     def prepare(self):
-        self.host.bb_host_prepare()
+        if self.case.precursor:
+            self.host.bb_host_prepare_turb()
+        else:
+            self.host.bb_host_prepare()
 
     def restart(self, path):
         """Continue from a restart file instead of prepare() (the state
         must be on the device already)."""
-        if self.host.bb_host_restart(str(path).encode()) != 0:
+        fn = (self.host.bb_host_restart_turb if self.case.precursor
+              else self.host.bb_host_restart)
+        if fn(str(path).encode()) != 0:
             raise RuntimeError("could not read restart file %s" % path)
 
     def write_restart(self, path):
-        """Pull the device state and write it as a restart file."""
+        """Pull the device state and write it as a restart file (the
+        precursor domain's layout for a precursor case)."""
         self.pull()
-        if self.host.bb_out_restart(str(path).encode()) != 0:
+        fn = (self.host.bb_out_restart_turb if self.case.precursor
+              else self.host.bb_out_restart)
+        if fn(str(path).encode()) != 0:
             raise RuntimeError("could not write restart file %s" % path)
 
     def write_vtk(self, directory, stepnum):
@@ -339,7 +358,10 @@ class Simulation:
         """One time step; returns the Lamb sub-iteration count.  Raises
         when the new dt fails the driver's blow-up test, as the reference
         ends the run there."""
-        it = self.host.bb_host_step()
+        if self.case.precursor:
+            it = self.host.bb_host_step_turb()
+        else:
+            it = self.host.bb_host_step()
         if it < 0:
             raise RuntimeError("the solution has diverged (dt = %g)"
                                % runtime.get_scalar("dt"))

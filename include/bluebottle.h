@@ -240,6 +240,9 @@ extern int init_cond;
 extern gradP_struct gradP;
 extern long int cpumem, gpumem;
 extern real pid_int, pid_back, Kp, Ki, Kd;
+/* the precursor domain's linear-forcing amplitude and integral scale
+ * (src/bluebottle.h:2342-2360) */
+extern real turbA, turbl;
 
 extern int *phase, *phase_shell;
 extern int **_phase, **_phase_shell;
@@ -510,16 +513,36 @@ real bb_lamb_err_noalloc(int np, int stride, real cut, real *const cur[6],
   real *const prev[6]);
 
 /* ---------------------------------------------------------------------
- * Outside this library's path.  Exported so that the reference driver
- * links unchanged; each prints which path it belongs to and exits with
- * EXIT_FAILURE.  Precursor inflow (src/bluebottle.h:2411, 2447, 2877,
- * 2926), the -DTEST checks (2616, 2628, 2753, 2766, 2780, 2792) and
- * helpers the step does not call (2704, 2717, 2943, 3341, 3355-3391).
+ * Precursor inflow and the precursor domain, on one GPU.
  * ------------------------------------------------------------------- */
+/* src/cuda_bluebottle.cu:536-1415 (src/bluebottle.h:2411, 2447): copy the
+ * inflow planes that bc_configs selects host -> device / device -> host */
 void cuda_dom_turb_planes_push(int *bc_configs);
 void cuda_dom_turb_planes_pull(int *bc_configs);
-void cuda_compute_turb_forcing(void);
+/* src/cuda_bluebottle.cu:4556-4820 (src/bluebottle.h:2926): sample the
+ * planes at bc_plane_pos out of the current field, plus vel */
 void cuda_yank_turb_planes(int *bc_flow_configs, real *bc_plane_pos, real *vel);
+/* src/cuda_bluebottle.cu:3817-3943 (src/bluebottle.h:2877): add the
+ * linear forcing turbA (k0/k) (u - <u>) to f_x, f_y, f_z, with k the
+ * kinetic energy of the cell-centred velocity and k0 = 13.5 turbA^2
+ * 0.0361 turbl^2 */
+void cuda_compute_turb_forcing(void);
+/* src/cuda_bluebottle.cu:3944-4017 (src/bluebottle.h:3341): k, half the
+ * mean over the cells of |u|^2 at the cell centres */
+real cuda_compute_energy(void);
+/* src/cuda_bluebottle.cu:4019-4120 (src/bluebottle.h:3355-3391): the face
+ * velocity averaged to the cell centres into a device array of Gcc.s3
+ * values (no ghosts); _dom is the device copy of the subdomain, _dom[0] */
+void cuda_colocate_Gfx(real *_u, real *_u_co, dom_struct *_dom);
+void cuda_colocate_Gfy(real *_v, real *_v_co, dom_struct *_dom);
+void cuda_colocate_Gfz(real *_w, real *_w_co, dom_struct *_dom);
+
+/* ---------------------------------------------------------------------
+ * Outside this library's path.  Exported so that the reference driver
+ * links unchanged; each prints which path it belongs to and exits with
+ * EXIT_FAILURE.  The -DTEST checks (src/bluebottle.h:2616, 2628, 2753,
+ * 2766, 2780, 2792) and helpers the step does not call (2704, 2717, 2943).
+ * ------------------------------------------------------------------- */
 void cuda_U_star_test_exp(void);
 void cuda_U_star_test_cos(void);
 void cuda_project_test(void);
@@ -529,10 +552,6 @@ void cuda_lamb_test(void);
 void cuda_PP_bicgstab_jacobi(void);
 void cuda_div_U(void);
 void cuda_collisions(void);
-real cuda_compute_energy(void);
-void cuda_colocate_Gfx(real *_u, real *_u_co, dom_struct *_dom);
-void cuda_colocate_Gfy(real *_v, real *_v_co, dom_struct *_dom);
-void cuda_colocate_Gfz(real *_w, real *_w_co, dom_struct *_dom);
 
 #ifdef __cplusplus
 }

@@ -66,6 +66,8 @@ typedef struct orc_state {
   /* precursor inflow planes, in the order of orc_plane_names (layouts:
    * W/E j + k*jnb, S/N k + i*knb, B/T i + j*inb of the component's grid) */
   real *plane[15];
+  /* the precursor domain's forcing amplitude and integral scale */
+  real turbA, turbl;
 } orc_state;
 
 /* Names of coeff[] entries, in order. */
@@ -108,6 +110,26 @@ real orc_find_dt(orc_state *s);
 void orc_store_u(orc_state *s);
 /* src/cuda_bluebottle.cu:3674-3814 (PID branch included). */
 void orc_compute_forcing(orc_state *s);
+
+/* --- the precursor domain (oracle_turb.c) --- */
+/* src/bluebottle_kernel.cu:3849-3889: component comp (0 u, 1 v, 2 w) at
+ * the cell centres into out[Gcc.s3] */
+void orc_colocate(const orc_state *s, int comp, real *out);
+/* src/cuda_bluebottle.cu:3944-4017 */
+real orc_compute_energy(orc_state *s);
+/* src/cuda_bluebottle.cu:3817-3942 */
+void orc_compute_turb_forcing(orc_state *s);
+/* src/rng.c: the state after rng_init(seed) from `state`, and one double
+ * (the state advanced in place) */
+unsigned long long orc_rng_seeded(unsigned long long state,
+  unsigned long long seed);
+double orc_rng_dbl(unsigned long long *state);
+/* src/domain.c:2856-3093 (field part), drawing from *rng_state */
+void orc_init_turb(orc_state *s, unsigned long long *rng_state);
+/* src/bluebottle.c:945-1016 and 1036-1081: the precursor's set-up after
+ * the pushes and one step */
+void orc_prepare_turb(orc_state *s);
+int orc_step_turb(orc_state *s);
 
 /* --- Poisson solve (oracle_poisson.c) --- */
 /* src/bicgstab_kernel.cu:483-562 */

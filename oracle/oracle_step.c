@@ -5,6 +5,8 @@
  * orc_prepare: src/bluebottle.c:445-545 (cages, first dt, BCs, interior).
  * orc_step:    src/bluebottle.c:598-711 (one time step with the Lamb
  *              sub-iteration), plus compute_vel_BC (src/domain.c:3144-3221).
+ * orc_prepare_turb, orc_step_turb: the precursor domain's set-up after the
+ *              pushes and one step (src/bluebottle.c:945-1016, 1036-1081).
  */
 #include <float.h>
 #include <math.h>
@@ -86,4 +88,39 @@ int orc_step(orc_state *s)
   s->dt0 = s->dt;
   s->dt = orc_find_dt(s);
   return iter;
+}
+
+void orc_prepare_turb(orc_state *s)
+{
+  orc_build_cages(s);
+  s->dt = orc_find_dt(s);
+  orc_dom_BC(s);
+}
+
+int orc_step_turb(orc_state *s)
+{
+  s->ttime += s->dt;
+  s->stepnum++;
+  orc_compute_forcing(s);
+  orc_compute_turb_forcing(s);
+  compute_vel_BC(s);
+  if(!s->implicit) orc_U_star_2(s);
+  else {
+    orc_star_helmholtz(s, 0);
+    orc_star_helmholtz(s, 1);
+    orc_star_helmholtz(s, 2);
+  }
+  orc_dom_BC_star(s);
+  orc_solvability(s);
+  orc_dom_BC_star(s);
+  orc_PP_bicgstab(s);
+  orc_dom_BC_phi(s);
+  orc_project(s);
+  orc_dom_BC(s);
+  orc_update_p(s);
+  orc_dom_BC_p(s);
+  orc_store_u(s);
+  s->dt0 = s->dt;
+  s->dt = orc_find_dt(s);
+  return 0;
 }

@@ -52,6 +52,7 @@ class orc_state(C.Structure):
         ("last_converged", C.c_int),
         ("implicit", C.c_int),
         ("plane", C.POINTER(C.c_double) * 15),
+        ("turbA", real), ("turbl", real),
     ]
 
 
@@ -72,7 +73,7 @@ FIELD_NAMES = [
 SCALARS = ["rho_f", "mu", "nu", "dt", "dt0", "ttime", "CFL", "pp_residual",
            "lamb_residual", "lamb_relax", "lamb_cut", "p_tDelay", "g_tDelay",
            "vel_tDelay", "Kp", "Ki", "Kd", "pid_int", "pid_back",
-           "interactionLengthRatio"]
+           "interactionLengthRatio", "turbA", "turbl"]
 INT_SCALARS = ["pp_max_iter", "lamb_max_iter", "out_plane", "stepnum",
                "init_cond"]
 
@@ -96,11 +97,20 @@ def load():
         lib.orc_array.argtypes = [C.POINTER(orc_state), C.c_char_p,
                                   C.POINTER(C.c_size_t), C.POINTER(C.c_int)]
         lib.orc_sizeof_state.restype = C.c_size_t
-        for fn in ("orc_find_dt", "orc_lamb_err", "orc_Nnm", "orc_Pnm"):
+        for fn in ("orc_find_dt", "orc_lamb_err", "orc_Nnm", "orc_Pnm",
+                   "orc_compute_energy"):
             getattr(lib, fn).restype = C.c_double
         lib.orc_Nnm.argtypes = [C.c_int, C.c_int]
         lib.orc_Pnm.argtypes = [C.c_int, C.c_int, C.c_double]
         lib.orc_step.restype = C.c_int
+        lib.orc_step_turb.restype = C.c_int
+        lib.orc_colocate.argtypes = [C.POINTER(orc_state), C.c_int, _PR]
+        lib.orc_rng_seeded.restype = C.c_ulonglong
+        lib.orc_rng_seeded.argtypes = [C.c_ulonglong, C.c_ulonglong]
+        lib.orc_rng_dbl.restype = C.c_double
+        lib.orc_rng_dbl.argtypes = [C.POINTER(C.c_ulonglong)]
+        lib.orc_init_turb.argtypes = [C.POINTER(orc_state),
+                                      C.POINTER(C.c_ulonglong)]
         lib.orc_bicgstab.restype = C.c_int
         lib.orc_bicgstab.argtypes = [C.POINTER(orc_state), _PR, _PR, C.c_int,
                                      C.c_double, _PR, _PI]

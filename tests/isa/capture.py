@@ -7,7 +7,10 @@ before and after, for replay against the gfx950 machine code (replay.py).
 
 Cases are the small parity cases of tests/test_gpu_parity.py (and
 `precursor` / `precursor_x`, the cases of tests/test_gpu_precursor.py:
-planes pushed, cuda_dom_BC, cuda_dom_BC_star, cuda_yank_turb_planes); the optional
+planes pushed, cuda_dom_BC, cuda_dom_BC_star, cuda_yank_turb_planes; and
+`turb`, the precursor domain: set-up, one step with its linear forcing,
+cuda_compute_energy and the three colocations, or `turb_forcing`, the
+same without the set-up and step); the optional
 fourth argument selects the Poisson solver form and `implicit` the
 Crank-Nicolson (Helmholtz) predictor."""
 import os
@@ -58,6 +61,33 @@ def main():
                                d.zs + 0.43 * d.zl)
         vel = (C.c_double * 18)(*[0.01 * (q + 1) for q in range(18)])
         s.hip.cuda_yank_turb_planes(cfg, pos, vel)
+        s.pull()
+        s.close()
+        print("captured", len([f for f in os.listdir(out) if f.endswith(".json")]),
+              "launches of", case_name, flush=True)
+        return
+    if case_name in ("turb", "turb_forcing"):
+        import ctypes as C
+        from test_oracle_turb import turb_case
+        case = turb_case(n=(12, 10, 8), hi=(1.0, 0.8, 0.7), turbA=0.6, seed=4)
+        s = sim.Simulation(case)
+        s.to_device()
+        if case_name == "turb":
+            s.prepare()
+            s.step()
+        else:
+            s.hip.cuda_compute_turb_forcing()
+        s.hip.cuda_compute_energy()
+        host = s.host
+        dst = C.POINTER(C.POINTER(C.c_double)).in_dll(host, "_phi")[0]
+        ddom = C.POINTER(C.c_void_p).in_dll(host, "_dom")[0]
+        for comp, fn in enumerate(("cuda_colocate_Gfx", "cuda_colocate_Gfy",
+                                   "cuda_colocate_Gfz")):
+            src = C.POINTER(C.POINTER(C.c_double)).in_dll(host,
+                                                          "_" + "uvw"[comp])[0]
+            f = getattr(s.hip, fn)
+            f.argtypes = [C.c_void_p] * 3
+            f(C.cast(src, C.c_void_p), C.cast(dst, C.c_void_p), ddom)
         s.pull()
         s.close()
         print("captured", len([f for f in os.listdir(out) if f.endswith(".json")]),

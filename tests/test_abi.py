@@ -122,18 +122,18 @@ def test_library_loads_without_gpu():
 
 
 def test_outside_path_entry_points_exit_with_a_message():
-    """The precursor, -DTEST and unused helpers are exported so the
-    reference driver links (src/bluebottle.c, src/precursor.c); calling one
-    ends the process with a message naming it, without touching a GPU."""
+    """The -DTEST checks and unused helpers are exported so the reference
+    driver links (src/bluebottle.c); calling one ends the process with a
+    message naming it, without touching a GPU."""
     code = ("import sys; sys.path.insert(0, %r)\n"
             "from bluebottle_amd import runtime\n"
             "host, hip = runtime.load()\n"
-            "hip.cuda_compute_turb_forcing()\n"
+            "hip.cuda_collisions()\n"
             "print('returned')\n") % ROOT
     p = subprocess.run([sys.executable, "-c", code], capture_output=True,
                        text=True, timeout=120)
     assert p.returncode == 1, (p.returncode, p.stdout, p.stderr)
-    assert "cuda_compute_turb_forcing" in p.stderr
+    assert "cuda_collisions" in p.stderr
     assert "outside this library's path" in p.stderr
     assert "returned" not in p.stdout
 

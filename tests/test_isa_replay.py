@@ -87,3 +87,38 @@ def test_precursor_kernels_machine_code_replays_clean(tmp_path):
                        timeout=1200)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
     assert "replayed 21 launches, 0 with findings" in r.stdout
+
+
+def test_precursor_domain_kernels_machine_code_replays_clean(tmp_path):
+    """The precursor domain's linear forcing (k_turb_sums, k_turb_final,
+    k_turb_apply), cuda_compute_energy's reduction and the three
+    colocations, replayed the same way."""
+    from test_emu import _asan
+    from bluebottle_amd import runtime
+    asan = _asan()
+    if not os.path.isabs(asan):
+        pytest.skip("AddressSanitizer runtime not available")
+    assert runtime.libs_present(), "build the library first"
+    subprocess.run(["make", "-s", "-j8", "-C", EMU], check=True)
+    cap = tmp_path / "cap"
+    env = dict(os.environ)
+    env.update(LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0",
+               BB_EMU_ISA_KERNELS="k_turb_sums,k_turb_final,k_turb_apply,"
+               "k_colocate", BB_EMU_ISA_MAX="8")
+    r = subprocess.run([sys.executable, os.path.join(ISA, "capture.py"),
+                        "turb_forcing", str(cap)], env=env,
+                       capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    launches = glob.glob(str(cap / "*.json"))
+    # forcing: sums, final, apply; energy: sums, final; three colocations
+    assert len(launches) == 8, len(launches)
+    env = dict(os.environ)
+    env["BB_ISA_OBJDIR"] = str(tmp_path / "co")
+    sys.path.insert(0, ISA)
+    import replay
+    replay.extract_code_objects(str(tmp_path / "co"))
+    r = subprocess.run([sys.executable, os.path.join(ISA, "replay.py"),
+                        str(cap)], env=env, capture_output=True, text=True,
+                       timeout=1200)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    assert "replayed 8 launches, 0 with findings" in r.stdout
