@@ -68,7 +68,9 @@ void bb_ensure_particle_workspace(void)
       BB_CHECK(hipFree(g_bb.bin_ind)); BB_CHECK(hipFree(g_bb.bin_bin));
       BB_CHECK(hipFree(g_bb.bin_start)); BB_CHECK(hipFree(g_bb.bin_end));
     }
-    BB_CHECK(hipMalloc(&g_bb.bin_ind, (size_t) np * 2 * sizeof(int)));
+    /* sorted indices, their bins, each particle's slot, the scattered
+     * particles */
+    BB_CHECK(hipMalloc(&g_bb.bin_ind, (size_t) np * 4 * sizeof(int)));
     BB_CHECK(hipMalloc(&g_bb.bin_bin, (size_t) np * sizeof(int)));
     BB_CHECK(hipMalloc(&g_bb.bin_start, (size_t) nb * sizeof(int)));
     BB_CHECK(hipMalloc(&g_bb.bin_end, (size_t) nb * sizeof(int)));
@@ -295,7 +297,13 @@ k_face_edge(int *__restrict__ f, int s1, int s2, int n0, int n1, int n2,
 }
 
 /* list of faces that part_BC writes: flag < 1 with a solid neighbour;
- * the transverse range includes the ghost rows, as in the reference */
+ * the transverse range includes the ghost rows, as in the reference.
+ * Each wavefront claims the slots of all its listed faces with one atomic
+ * (its lowest listing lane adds the ballot's population count); a lane's
+ * slot is that base plus the listing lanes below it.  The loop runs the
+ * same number of times in every lane of a block, so every lane takes part
+ * in each ballot.  The list order is whatever the atomics give, as
+ * before: part_BC writes each face independently. */
 __global__ void __launch_bounds__(256)
 k_face_list(BBGeom G, int comp, const int *__restrict__ ph,
   const int *__restrict__ flag, int *__restrict__ list, int *__restrict__ count,
@@ -305,20 +313,33 @@ k_face_list(BBGeom G, int comp, const int *__restrict__ ph,
   if(comp == 0) { ni = G.xn + 1; nj = G.yn + 2; nk = G.zn + 2; s1 = G.us1; s2 = G.us2; off_i = 1; off_j = 0; off_k = 0; cs = 1; }
   else if(comp == 1) { ni = G.xn + 2; nj = G.yn + 1; nk = G.zn + 2; s1 = G.vs1; s2 = G.vs2; off_i = 0; off_j = 1; off_k = 0; cs = G.cs1; }
   else { ni = G.xn + 2; nj = G.yn + 2; nk = G.zn + 1; s1 = G.ws1; s2 = G.ws2; off_i = 0; off_j = 0; off_k = 1; cs = G.cs2; }
-  long total = (long) ni * nj * nk;
-  long stride = (long) gridDim.x * 256;
-  for(long q = (long) blockIdx.x * 256 + threadIdx.x; q < total; q += stride) {
-    int i = (int) (q % ni) + off_i;
-    int j = (int) ((q / ni) % nj) + off_j;
-    int k = (int) (q / ((long) ni * nj)) + off_k;
-    int C = i + j*s1 + k*s2;
-    if(flag[C] >= 1) continue;
-    int Cc = i + j*G.cs1 + k*G.cs2;
-    int p0 = ph[Cc - cs], p1 = ph[Cc];
-    int P = p0 > -1 ? p0 : p1;
-    if(P < 0) continue;
-    int slot = atomicAdd(count, 1);
-    if(slot < cap) { list[2*slot] = C; list[2*slot+1] = P; }
+  const long total = (long) ni * nj * nk;
+  const long stride = (long) gridDim.x * 256;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for(long base = (long) blockIdx.x * 256; base < total; base += stride) {
+    const long q = base + threadIdx.x;
+    int C = 0, P = -1;
+    if(q < total) {
+      const int i = (int) (q % ni) + off_i;
+      const int j = (int) ((q / ni) % nj) + off_j;
+      const int k = (int) (q / ((long) ni * nj)) + off_k;
+      C = i + j*s1 + k*s2;
+      if(flag[C] < 1) {
+        const int Cc = i + j*G.cs1 + k*G.cs2;
+        const int p0 = ph[Cc - cs], p1 = ph[Cc];
+        P = p0 > -1 ? p0 : p1;
+      }
+    }
+    const unsigned long long mask = __ballot(P >= 0);
+    const int leader = mask ? __ffsll((unsigned long long) mask) - 1 : 0;
+    int slot = 0;
+    if(mask && lane == leader) slot = atomicAdd(count, __popcll(mask));
+    slot = __shfl(slot, leader, 64);
+    if(P >= 0) {
+      slot += __popcll(mask & below);
+      if(slot < cap) { list[2*slot] = C; list[2*slot+1] = P; }
+    }
   }
 }
 
@@ -841,10 +862,13 @@ __device__ void pair_force(const MoveArgs &M, part_struct *parts, int i, int j)
   Pi->iLz += Loz;
 }
 
-/* bins: per-particle bin, counts (int atomics), then a one-block scan and
- * a stable placement (rank = earlier particles in the same bin) */
+/* bins: per-particle bin and a slot in it (int atomics on the counts),
+ * a one-block scan, the particles scattered to their slots, then each
+ * placed at its rank among the bin's particles by index (the order the
+ * reference's sort by bin gives); the rank is a count over the bin's own
+ * particles only */
 __global__ void k_bin_fill(int n, part_struct *parts, dom_struct bd,
-  int *__restrict__ pbin, int *__restrict__ cnt)
+  int *__restrict__ pbin, int *__restrict__ cnt, int *__restrict__ slot)
 {
   int pp = blockIdx.x * blockDim.x + threadIdx.x;
   if(pp >= n) return;
@@ -861,7 +885,7 @@ __global__ void k_bin_fill(int n, part_struct *parts, dom_struct bd,
   int c = ib + jb*bd.Gcc.s1 + kb*bd.Gcc.s2;
   pbin[pp] = c;
   parts[pp].bin = c;
-  atomicAdd(&cnt[c], 1);
+  slot[pp] = atomicAdd(&cnt[c], 1);
 }
 
 __global__ void k_bin_scan(int nb, const int *__restrict__ cnt,
@@ -889,15 +913,27 @@ __global__ void k_bin_scan(int nb, const int *__restrict__ cnt,
   }
 }
 
-__global__ void k_bin_place(int n, const int *__restrict__ pbin,
-  const int *__restrict__ start, int *__restrict__ ind, int *__restrict__ sbin)
+/* particle i to position start[bin] + its slot, in the bin's segment of
+ * `tmp` (slots are unique within a bin, so every position is written once) */
+__global__ void k_bin_scatter(int n, const int *__restrict__ pbin,
+  const int *__restrict__ slot, const int *__restrict__ start,
+  int *__restrict__ tmp)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= n) return;
-  int b = pbin[i];
+  tmp[start[pbin[i]] + slot[i]] = i;
+}
+
+__global__ void k_bin_place(int n, const int *__restrict__ pbin,
+  const int *__restrict__ start, const int *__restrict__ end,
+  const int *__restrict__ tmp, int *__restrict__ ind, int *__restrict__ sbin)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const int b = pbin[i];
   int rank = 0;
-  for(int q = 0; q < i; q++) rank += (pbin[q] == b);
-  int pos = start[b] + rank;
+  for(int s = start[b]; s < end[b]; s++) rank += (tmp[s] < i);
+  const int pos = start[b] + rank;
   ind[pos] = i;
   sbin[pos] = b;
 }
@@ -1177,13 +1213,17 @@ static void move_common(int final_)
     int nbins = binDom.Gcc.s3;
     BB_CHECK(hipMemsetAsync(g_bb.bin_end, 0, (size_t) nbins * sizeof(int), st));
     /* counts accumulate in bin_end, the scan writes start/end */
-    k_bin_fill<<<nb, th, 0, st>>>(nparts, P, binDom, g_bb.bin_bin, g_bb.bin_end);
+    int *slot = g_bb.bin_ind + 2 * nparts, *tmp = g_bb.bin_ind + 3 * nparts;
+    k_bin_fill<<<nb, th, 0, st>>>(nparts, P, binDom, g_bb.bin_bin, g_bb.bin_end,
+      slot);
     k_bin_scan<<<1, 1024, 0, st>>>(nbins, g_bb.bin_end, g_bb.bin_start,
       g_bb.bin_end);
     /* partBin by particle is in bin_bin; sorted arrays go to bin_ind
      * (indices) and bin_ind + nparts (bins) */
+    k_bin_scatter<<<nb, th, 0, st>>>(nparts, g_bb.bin_bin, slot,
+      g_bb.bin_start, tmp);
     k_bin_place<<<nb, th, 0, st>>>(nparts, g_bb.bin_bin, g_bb.bin_start,
-      g_bb.bin_ind, g_bb.bin_ind + nparts);
+      g_bb.bin_end, tmp, g_bb.bin_ind, g_bb.bin_ind + nparts);
     k_collision_parts<<<nb, th, 0, st>>>(M, nparts, P, binDom, g_bb.bin_start,
       g_bb.bin_end, g_bb.bin_ind + nparts, g_bb.bin_ind);
     k_spring<<<nb, th, 0, st>>>(nparts, P);

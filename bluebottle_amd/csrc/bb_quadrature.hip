@@ -386,7 +386,49 @@ extern "C" void cuda_Lamb(void)
 }
 
 /* per-particle error of src/quadrature_kernel.cu:578-654, each particle
- * in its own scratch (lane 0 runs the selection) */
+ * in its own scratch (one wavefront per particle).
+ *
+ * The reference repeatedly picks the coefficient of largest square (the
+ * lowest index among equal squares; only squares above FLT_MIN qualify),
+ * scores it against its previous value -- relative to the first pick,
+ * coefficients at most lamb_cut of it score 0 -- and zeroes it, 6*stride
+ * times; the result is the largest score (at least FLT_MIN).  A pick's
+ * score depends on the order only through the first pick, so every
+ * qualifying coefficient is scored at once here, the first pick found by
+ * an argmax over the wavefront.  The picks past the qualifying ones see
+ * only zeroes and re-select the last pick (the smallest square, the
+ * highest index among equal ones), scoring a zero value there; with none
+ * qualifying, index 0 is picked first.  Each score is the reference's
+ * expression, so the result is the same double. */
+struct ErrPick { double sq; int j; };
+
+/* a before b in the reference's order of picks (j < 0: no pick) */
+__device__ inline bool pick_first(ErrPick a, ErrPick b)
+{
+  if(b.j < 0) return a.j >= 0;
+  if(a.j < 0) return false;
+  return a.sq > b.sq || (a.sq == b.sq && a.j < b.j);
+}
+
+__device__ inline ErrPick shfl_pick(ErrPick p, int d)
+{
+  ErrPick o;
+  o.sq = __shfl_down(p.sq, d, 64);
+  o.j = __shfl_down(p.j, d, 64);
+  return o;
+}
+
+__device__ inline double lamb_score(double val, double old, double cut,
+  double first)
+{
+  if(fabs(val) > cut * fabs(first)) {
+    double div = fabs(val);
+    if(div < 1e-16) div = 1e-16;
+    return fabs((val - old) / div);
+  }
+  return 0.;
+}
+
 __global__ void __launch_bounds__(64)
 k_lamb_err(int stride, double lamb_cut, const double *pre, const double *pre0,
   const double *pim, const double *pim0, const double *fre, const double *fre0,
@@ -394,9 +436,12 @@ k_lamb_err(int stride, double lamb_cut, const double *pre, const double *pre0,
   const double *cim, const double *cim0, double *__restrict__ out)
 {
   __shared__ double sc[6*21], sc0[6*21];
+  __shared__ double s_first;
+  __shared__ int s_last, s_count;
   const int part = blockIdx.x;
+  const int lane = threadIdx.x;
   const int n6 = 6 * stride;
-  for(int q = threadIdx.x; q < stride; q += blockDim.x) {
+  for(int q = lane; q < stride; q += 64) {
     int o = part*stride + q;
     sc[q] = pre[o]; sc[q + stride] = pim[o]; sc[q + 2*stride] = fre[o];
     sc[q + 3*stride] = fim[o]; sc[q + 4*stride] = cre[o]; sc[q + 5*stride] = cim[o];
@@ -405,27 +450,48 @@ k_lamb_err(int stride, double lamb_cut, const double *pre, const double *pre0,
     sc0[q + 5*stride] = cim0[o];
   }
   __syncthreads();
-  if(threadIdx.x != 0) return;
-  int loc = 0;
-  double first = 0.;
-  double best = FLT_MIN;
-  for(int i = 0; i < n6; i++) {
-    double tmp = FLT_MIN;
-    for(int j = 0; j < n6; j++) {
-      if(sc[j]*sc[j] > tmp) { tmp = sc[j]*sc[j]; loc = j; }
-    }
-    double val = sc[loc];
-    if(i == 0) first = val;
-    double err;
-    if(fabs(val) > lamb_cut*fabs(first)) {
-      double div = fabs(val);
-      if(div < 1e-16) div = 1e-16;
-      err = fabs((val - sc0[loc]) / div);
-    } else err = 0.;
-    if(err > best) best = err;
-    sc[loc] = 0.;
+  /* this lane's coefficients j = lane, lane + 64 (n6 <= 126) */
+  ErrPick first = {0., -1}, last = {0., -1};
+  int count = 0;
+  for(int j = lane; j < n6; j += 64) {
+    const double sq = sc[j] * sc[j];
+    if(!(sq > (double) FLT_MIN)) continue;
+    const ErrPick p = {sq, j};
+    count++;
+    if(pick_first(p, first)) first = p;
+    if(last.j < 0 || pick_first(last, p)) last = p;
   }
-  out[part] = best;
+  for(int d = 32; d > 0; d >>= 1) {
+    const ErrPick f = shfl_pick(first, d), l = shfl_pick(last, d);
+    count += __shfl_down(count, d, 64);
+    if(pick_first(f, first)) first = f;
+    if(l.j >= 0 && (last.j < 0 || pick_first(last, l))) last = l;
+  }
+  if(lane == 0) {
+    s_count = count;
+    s_first = count > 0 ? sc[first.j] : sc[0];
+    s_last = count > 0 ? last.j : 0;
+  }
+  __syncthreads();
+  const double f0 = s_first;
+  const int nq = s_count;
+  double best = FLT_MIN;
+  for(int j = lane; j < n6; j += 64) {
+    const bool picked = nq > 0 ? sc[j] * sc[j] > (double) FLT_MIN : j == 0;
+    if(!picked) continue;
+    const double e = lamb_score(sc[j], sc0[j], lamb_cut, f0);
+    if(e > best) best = e;
+  }
+  /* the remaining picks: a zero value at the last pick */
+  if(lane == 0 && n6 > (nq > 0 ? nq : 1)) {
+    const double e = lamb_score(0., sc0[s_last], lamb_cut, f0);
+    if(e > best) best = e;
+  }
+  for(int d = 32; d > 0; d >>= 1) {
+    const double o = __shfl_down(best, d, 64);
+    if(o > best) best = o;
+  }
+  if(lane == 0) out[part] = best;
 }
 
 extern "C" real cuda_lamb_err(void)

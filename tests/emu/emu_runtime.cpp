@@ -151,6 +151,20 @@ double shfl_down_d(double v, unsigned d, int w) { return shfl(g_blk.xd, v, d, w)
 float shfl_down_f(float v, unsigned d, int w) { return shfl(g_blk.xf, v, d, w); }
 int shfl_down_i(int v, unsigned d, int w) { return shfl(g_blk.xi, v, d, w); }
 double shfl_idx_d(double v, int src, int w) { return shfl_idx(g_blk.xd, v, src, w); }
+int shfl_idx_i(int v, int src, int w) { return shfl_idx(g_blk.xi, v, src, w); }
+
+unsigned long long ballot(int p)
+{
+  int tid = flat_tid();
+  g_blk.xi[tid] = p != 0;
+  g_blk.bar->arrive_and_wait();
+  int base = tid - tid % 64;
+  unsigned long long m = 0;
+  for(int l = 0; l < 64 && base + l < g_blk.nthreads; l++)
+    if(g_blk.xi[base + l]) m |= 1ull << l;
+  g_blk.bar->arrive_and_wait();
+  return m;
+}
 
 bool is_device_range(const void *p, size_t bytes)
 {

@@ -115,6 +115,9 @@ double shfl_down_d(double v, unsigned delta, int width);
 float shfl_down_f(float v, unsigned delta, int width);
 int shfl_down_i(int v, unsigned delta, int width);
 double shfl_idx_d(double v, int src, int width);
+int shfl_idx_i(int v, int src, int width);
+/* bit l set when lane l of the caller's 64-lane wavefront passes p != 0 */
+unsigned long long ballot(int p);
 /* true when [p, p+bytes) lies inside one device (or pinned host)
  * allocation; null passes */
 bool is_device_range(const void *p, size_t bytes);
@@ -135,6 +138,10 @@ inline double __shfl_down(double v, unsigned d, int w = 64) { return bb_emu::shf
 inline float __shfl_down(float v, unsigned d, int w = 64) { return bb_emu::shfl_down_f(v, d, w); }
 inline int __shfl_down(int v, unsigned d, int w = 64) { return bb_emu::shfl_down_i(v, d, w); }
 inline double __shfl(double v, int src, int w = 64) { return bb_emu::shfl_idx_d(v, src, w); }
+inline int __shfl(int v, int src, int w = 64) { return bb_emu::shfl_idx_i(v, src, w); }
+inline unsigned long long __ballot(int p) { return bb_emu::ballot(p); }
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long) x); }
 
 inline int atomicAdd(int *a, int v) { return __atomic_fetch_add(a, v, __ATOMIC_SEQ_CST); }
 inline int atomicExch(int *a, int v) { return __atomic_exchange_n(a, v, __ATOMIC_SEQ_CST); }

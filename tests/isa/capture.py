@@ -10,7 +10,9 @@ Cases are the small parity cases of tests/test_gpu_parity.py (and
 planes pushed, cuda_dom_BC, cuda_dom_BC_star, cuda_yank_turb_planes; and
 `turb`, the precursor domain: set-up, one step with its linear forcing,
 cuda_compute_energy and the three colocations, or `turb_forcing`, the
-same without the set-up and step); the optional
+same without the set-up and step; `particle_kernels`, three spheres with
+a pair 0.3 r apart: the cages and face lists of the set-up, one
+cuda_move_parts_sub (bins) and cuda_Lamb then cuda_lamb_err); the optional
 fourth argument selects the Poisson solver form and `implicit` the
 Crank-Nicolson (Helmholtz) predictor."""
 import os
@@ -88,6 +90,24 @@ def main():
             f = getattr(s.hip, fn)
             f.argtypes = [C.c_void_p] * 3
             f(C.cast(src, C.c_void_p), C.cast(dst, C.c_void_p), ddom)
+        s.pull()
+        s.close()
+        print("captured", len([f for f in os.listdir(out) if f.endswith(".json")]),
+              "launches of", case_name, flush=True)
+        return
+    if case_name == "particle_kernels":
+        a = 0.15
+        parts = [sim.Particle(x=0.3, y=0.5, z=0.5, r=a),
+                 sim.Particle(x=0.3 + 2.3 * a, y=0.5, z=0.5, r=a),
+                 sim.Particle(x=0.75, y=0.2, z=0.8, r=a)]
+        case = sim.Case(n=(12, 12, 12), nu=0.01, init="taylor_green", seed=3,
+                        noise=1e-3, particles=parts, lamb_max_iter=1)
+        s = sim.Simulation(case)
+        s.to_device()
+        s.prepare()
+        s.hip.cuda_move_parts_sub()
+        s.hip.cuda_Lamb()
+        s.hip.cuda_lamb_err()
         s.pull()
         s.close()
         print("captured", len([f for f in os.listdir(out) if f.endswith(".json")]),

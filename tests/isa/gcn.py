@@ -1161,6 +1161,15 @@ class Exec:
         idx = int(np.argmax(act)) if np.any(act) else 0
         self.sset(w, o[0], int(self._u(w, o[1])[idx]))
 
+    def op_bcnt_u32_b32(self, w, o, m, act):
+        # D = popcount(S0) + S1
+        x = self._u(w, o[1]).astype(np.uint32)
+        x = x - ((x >> np.uint32(1)) & np.uint32(0x55555555))
+        x = (x & np.uint32(0x33333333)) + ((x >> np.uint32(2)) & np.uint32(0x33333333))
+        x = (x + (x >> np.uint32(4))) & np.uint32(0x0F0F0F0F)
+        cnt = ((x * np.uint32(0x01010101)) >> np.uint32(24)).astype(np.uint32)
+        self.vset(w, o[0], cnt + self._u(w, o[2]))
+
     def op_mbcnt_lo_u32_b32(self, w, o, m, act):
         msk = self.sv(w, o[1]) & M32
         c = self._u(w, o[2])

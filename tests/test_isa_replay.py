@@ -6,9 +6,11 @@ unwritten scalar register, results equal to the emulation's.
 An earlier k_bc_axis, which selected kernel-argument fields by a run-time
 axis index, compiled to code that loaded an extent through a scalar
 register pair its axis-2 path never wrote; this replay is how that was
-found, and it runs here so that such code cannot return unnoticed.  (The
-full replay of every kernel of the step takes tens of minutes and is run
-by hand: tests/isa/capture.py, tests/isa/replay.py.)"""
+found, and it runs here so that such code cannot return unnoticed.  The
+precursor kernels, the precursor domain's forcing and the particle list,
+bin and error kernels are replayed the same way.  (The full replay of
+every kernel of the step takes tens of minutes and is run by hand:
+tests/isa/capture.py, tests/isa/replay.py.)"""
 import glob
 import json
 import os
@@ -122,3 +124,40 @@ def test_precursor_domain_kernels_machine_code_replays_clean(tmp_path):
                        timeout=1200)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
     assert "replayed 8 launches, 0 with findings" in r.stdout
+
+
+def test_particle_list_and_bin_kernels_machine_code_replays_clean(tmp_path):
+    """The wave-aggregated face lists (k_face_list: one atomic per
+    wavefront, the lists compared as multisets of records), the bin
+    placement (k_bin_fill's slots, k_bin_scatter, k_bin_place's rank within
+    the bin) and the wavefront-wide k_lamb_err, on three spheres with a pair
+    0.3 r apart."""
+    from test_emu import _asan
+    from bluebottle_amd import runtime
+    asan = _asan()
+    if not os.path.isabs(asan):
+        pytest.skip("AddressSanitizer runtime not available")
+    assert runtime.libs_present(), "build the library first"
+    subprocess.run(["make", "-s", "-j8", "-C", EMU], check=True)
+    cap = tmp_path / "cap"
+    env = dict(os.environ)
+    env.update(LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0",
+               BB_EMU_ISA_KERNELS="k_face_list,k_bin_fill,k_bin_scatter,"
+               "k_bin_place,k_lamb_err", BB_EMU_ISA_MAX="4")
+    r = subprocess.run([sys.executable, os.path.join(ISA, "capture.py"),
+                        "particle_kernels", str(cap)], env=env,
+                       capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    launches = glob.glob(str(cap / "*.json"))
+    # three face lists, one bin pass of three kernels, one k_lamb_err
+    assert len(launches) == 7, len(launches)
+    env = dict(os.environ)
+    env["BB_ISA_OBJDIR"] = str(tmp_path / "co")
+    sys.path.insert(0, ISA)
+    import replay
+    replay.extract_code_objects(str(tmp_path / "co"))
+    r = subprocess.run([sys.executable, os.path.join(ISA, "replay.py"),
+                        str(cap)], env=env, capture_output=True, text=True,
+                       timeout=1200)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    assert "replayed 7 launches, 0 with findings" in r.stdout

@@ -333,6 +333,48 @@ def test_bicgstab_true_residual():
     assert resid.value <= 1e-10
 
 
+@pytest.mark.parametrize("which", ["channel", "particles"])
+def test_bicgstab_iterates_match_textbook_implementation(which):
+    """The iterate sequence, not just the converged answer: after each of
+    the first eight iterations, orc_bicgstab's x equals that of SciPy's
+    BiCGSTAB (van der Vorst's right-preconditioned method, r-hat = r0, the
+    algorithm CUSP implements) run on the same matrix -- assembled here
+    column by column from the oracle's operator -- with the same Jacobi
+    preconditioner, b and x0.  Different summation orders and update
+    spellings leave differences at rounding level."""
+    import ctypes as C
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    case = (configs.channel_case(n=(8, 6, 6)) if which == "channel" else
+            configs.small_particle_case(n=8, nparts=1, radius_dx=2.0))
+    o, d = state(case)
+    o.call("build_cages")
+    n = d.xn * d.yn * d.zn
+    cols = [_matvec(o, e.copy()) for e in np.eye(n)]
+    A = sp.csr_matrix(np.array(cols).T)
+    dinv = 1.0 / A.diagonal()
+    M = spla.LinearOperator((n, n), matvec=lambda v: dinv * v)
+    rng = np.random.default_rng(21)
+    b = rng.normal(size=n)
+    x0 = rng.normal(size=n) * 0.1
+    ref = []
+    spla.bicgstab(A, b, x0=x0.copy(), rtol=1e-300, atol=0.0, maxiter=8, M=M,
+                  callback=lambda xk: ref.append(xk.copy()))
+    assert len(ref) == 8
+    P = C.POINTER(C.c_double)
+    for k in range(1, 9):
+        x = x0.copy()
+        resid, conv = C.c_double(), C.c_int()
+        it = o.lib.orc_bicgstab(o.ptr, b.ctypes.data_as(P), x.ctypes.data_as(P),
+                                k, 1e-300, C.byref(resid), C.byref(conv))
+        assert it == k
+        err = np.abs(x - ref[k - 1]).max() / np.abs(ref[k - 1]).max()
+        assert err < 1e-11, (k, err)
+        # the recursively updated residual is the true one, to rounding
+        assert resid.value == pytest.approx(np.linalg.norm(b - A @ x),
+                                            rel=1e-8)
+
+
 @pytest.mark.parametrize("which", ["tg", "channel", "particles"])
 def test_projection_is_divergence_free(which):
     if which == "tg":

@@ -312,3 +312,33 @@ def test_store_u_implicit_keeps_no_diffusion_history():
     assert np.array_equal(o.array("conv0_u"), o.array("conv_u"))
     assert np.array_equal(o.array("u0"), o.array("u"))
     assert np.all(o.array("diff0_u") == 7.0)
+
+
+def test_pcg_iterates_match_textbook_implementation():
+    """The iterate sequence: after each of the first six iterations,
+    orc_pcg's x equals that of SciPy's preconditioned conjugate gradient
+    (the method CUSP's cg implements) on the same assembled matrix, with
+    the same Jacobi preconditioner and b, from x0 = 0, for a tangential
+    and a wall-normal component of a channel."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    o, d = state(configs.channel_case(n=(6, 4, 6), nparts=0))
+    o.s.nu = 0.05
+    o.s.dt = 0.05
+    rng = np.random.default_rng(12)
+    for comp in (0, 1):
+        vals = _coeffs(o, comp)
+        A = sp.csr_matrix(_dense(o, comp, vals))
+        n = A.shape[0]
+        dinv = 1.0 / A.diagonal()
+        M = spla.LinearOperator((n, n), matvec=lambda v: dinv * v)
+        b = rng.normal(size=n)
+        ref = []
+        spla.cg(A, b, x0=np.zeros(n), rtol=1e-300, atol=0.0, maxiter=6, M=M,
+                callback=lambda xk: ref.append(xk.copy()))
+        assert len(ref) == 6
+        for k in range(1, 7):
+            x, it, resid, conv = _pcg(o, comp, vals, b, k, 1e-300)
+            assert it == k
+            err = np.abs(x - ref[k - 1]).max() / np.abs(ref[k - 1]).max()
+            assert err < 1e-12, (comp, k, err)
