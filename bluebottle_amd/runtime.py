@@ -46,6 +46,12 @@ ENTRY_POINTS = {
     "cuda_dom_turb_planes_push": (None, [C.POINTER(C.c_int)]),
     "cuda_dom_turb_planes_pull": (None, [C.POINTER(C.c_int)]),
     "cuda_yank_turb_planes": (None, [C.POINTER(C.c_int), _PD, _PD]),
+    "cuda_compute_turb_forcing": (None, []),
+    "cuda_compute_energy": (C.c_double, []),
+    # source, destination and subdomain are device pointers
+    "cuda_colocate_Gfx": (None, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cuda_colocate_Gfy": (None, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cuda_colocate_Gfz": (None, [C.c_void_p, C.c_void_p, C.c_void_p]),
     # node tables and node values are device pointers
     "cuda_quad_check_nodes": (None, [C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
     "cuda_quad_interp": (None, [C.c_int, C.c_void_p, C.c_void_p, C.c_int,

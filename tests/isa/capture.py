@@ -87,9 +87,8 @@ def main():
                                    "cuda_colocate_Gfz")):
             src = C.POINTER(C.POINTER(C.c_double)).in_dll(host,
                                                           "_" + "uvw"[comp])[0]
-            f = getattr(s.hip, fn)
-            f.argtypes = [C.c_void_p] * 3
-            f(C.cast(src, C.c_void_p), C.cast(dst, C.c_void_p), ddom)
+            getattr(s.hip, fn)(C.cast(src, C.c_void_p), C.cast(dst, C.c_void_p),
+                               ddom)
         s.pull()
         s.close()
         print("captured", len([f for f in os.listdir(out) if f.endswith(".json")]),

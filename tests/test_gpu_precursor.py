@@ -102,7 +102,6 @@ def _turb_state(seed, turbA=0.6, n=(12, 10, 8)):
 def test_compute_energy():
     s = _turb_state(21)
     o = OracleState.from_sim(s)
-    s.hip.cuda_compute_energy.restype = C.c_double
     k = s.hip.cuda_compute_energy()
     want = o.lib.orc_compute_energy(o.ptr)
     assert k == pytest.approx(want, rel=1e-13)
@@ -138,9 +137,8 @@ def test_colocate(comp, name):
     src = C.POINTER(C.POINTER(C.c_double)).in_dll(s.host, "_" + "uvw"[comp])[0]
     dst = C.POINTER(C.POINTER(C.c_double)).in_dll(s.host, "_phi")[0]
     ddom = C.POINTER(C.c_void_p).in_dll(s.host, "_dom")[0]
-    fn = getattr(s.hip, name)
-    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
-    fn(C.cast(src, C.c_void_p), C.cast(dst, C.c_void_p), ddom)
+    getattr(s.hip, name)(C.cast(src, C.c_void_p), C.cast(dst, C.c_void_p),
+                         ddom)
     s.pull()
     want = np.zeros(d.Gcc.s3)
     o.lib.orc_colocate(o.ptr, comp, want.ctypes.data_as(C.POINTER(C.c_double)))
