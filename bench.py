@@ -369,11 +369,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--nparts", type=int, default=1000)
-    ap.add_argument("--workload", choices=("sediment", "tg"),
+    ap.add_argument("--workload", choices=("sediment", "tg", "channel"),
                     default="sediment",
                     help="sediment: 256^3 with --nparts spheres (the metric's "
                          "configuration); tg: the same box as a sphere-free "
-                         "Taylor-Green flow")
+                         "Taylor-Green flow; channel: the dense sheared "
+                         "channel, 384x192x384 with 8000 spheres")
     ap.add_argument("--modes", default="fp64,mixed,learned")
     ap.add_argument("--learn-collect-steps", type=int, default=3,
                     help="mixed steps whose inner solves supply the training "
@@ -418,6 +419,12 @@ def main():
         desc = ("sedimentation %dx%dx%d periodic, %d spheres r=6dx, "
                 "rho_p/rho_f=2, Re_p=10, L=4, 26 nodes, Poisson rel tol 1e-8"
                 % (args.n, args.n, args.n, args.nparts))
+    elif workload == "channel":
+        case = configs.config5()
+        desc = ("dense sheared channel 384x192x384, y walls at -U/2 and "
+                "+U/2, periodic x and z, Re_bulk=2000, %d spheres r=4dx, "
+                "rho_p/rho_f=1.5, L=4, Poisson rel tol 1e-8"
+                % len(case.particles))
     else:
         case = configs.taylor_green(n=args.n, pp_residual=1e-8)
         desc = ("Taylor-Green %dx%dx%d periodic unit box, Re=100, no spheres, "
