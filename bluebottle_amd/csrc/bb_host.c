@@ -587,8 +587,14 @@ int bb_host_init_turb(unsigned long long seed)
 int domain_init_turb(void)
 {
   if(nsubdom > dev_end - dev_start + 1) return EXIT_FAILURE;
+  /* the driver reads part.config first (src/bluebottle.c:877-878); keep
+   * its (empty) particle list */
+  part_struct *keep = parts;
+  const int nkeep = nparts;
   bb_host_domain(Dom.xs, Dom.xe, Dom.xn, Dom.ys, Dom.ye, Dom.yn, Dom.zs,
     Dom.ze, Dom.zn);
+  parts = keep;
+  nparts = nkeep;
   bb_host_init_turb((unsigned long long) time(NULL));
   return EXIT_SUCCESS;
 }

@@ -382,3 +382,42 @@ def test_turb_restart_layout_and_round_trip(tmp_path):
         assert runtime.get_scalar("ttime") == 1.25
     finally:
         s.close()
+
+
+def test_driver_sequence_with_reference_files(tmp_path):
+    """The precursor rank's set-up under the reference's names, on its own
+    turb.config: turb_read_input, parts_read_input(1) (no particles in the
+    precursor), domain_init_turb (grids, arrays, the clock-seeded field):
+    the grid the file states, l = 8, a field of amplitude 3 turbA l / 2
+    with zero mean, and the first dt from the viscous limit alone."""
+    if sim.Simulation._alive is not None:
+        sim.Simulation._alive.close()
+    host, _ = runtime.load()
+    inp = tmp_path / "input"
+    inp.mkdir()
+    (inp / "turb.config").write_text(
+        open(os.path.join(REF_INPUT, "turb.config")).read())
+    (inp / "part.config").write_text(
+        open(os.path.join(REF_INPUT, "part.config")).read())
+    root = C.c_char * 256
+    old = root.in_dll(host, "bb_root_dir").value
+    root.in_dll(host, "bb_root_dir").value = str(tmp_path).encode()
+    try:
+        host.turb_read_input()
+        host.parts_read_input(1)
+        assert host.domain_init_turb() == 0
+        assert runtime.get_scalar("nparts", "i") == 0
+        d = runtime.dom()
+        assert (d.Gfx.s3b, d.Gcc.s3b) == (67 * 66 * 66, 66 ** 3)
+        assert runtime.get_scalar("turbl") == 8.0
+        u = runtime.array_view("u", d.Gfx.s3b)
+        assert np.abs(u).max() <= 3 * 3.0 * 8.0 / 2 * 1.01
+        assert np.abs(u).max() > 1.0
+        inner = u.reshape(d.Gfx.knb, d.Gfx.jnb, d.Gfx.inb)[1:-1, 1:-1, 1:-2]
+        assert abs(inner.mean()) < 1e-12 * np.abs(inner).max()
+        dx = 8.0 / 64
+        assert runtime.get_scalar("dt") == pytest.approx(0.5 / (2 * 0.5 / dx ** 2),
+                                                         rel=1e-15)
+    finally:
+        host.bb_host_free()
+        root.in_dll(host, "bb_root_dir").value = old
