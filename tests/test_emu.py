@@ -1,7 +1,8 @@
 """The HIP kernels, built for the host by tests/emu (g++ with
 AddressSanitizer and a bounds-checked HIP emulation), run one full step of
-a small particle case and of a wall-bounded case on the CPU and agree with
-the oracle, and so does the mixed form's two-sweep inner solve.  An
+a small particle case, of a wall-bounded case and of the precursor domain
+on the CPU and agree with the oracle, and so does the mixed form's
+two-sweep inner solve.  An
 out-of-bounds index, a host pointer passed to a kernel or an invalid
 launch shape aborts the child process with a report, so this guards every
 kernel change before it reaches a GPU.  The full GPU suite
@@ -66,6 +67,22 @@ for name, case in (
     assert e < 1e-8, (name, "two-sweep phi", e)
     s.close()
     print("emulated step", name, "ok", flush=True)
+
+# the precursor domain: its set-up and one step with the linear forcing
+from test_oracle_turb import turb_case
+s = sim.Simulation(turb_case(n=(8, 6, 5), turbA=0.5, seed=4))
+s.to_device()
+o = OracleState.from_sim(s)
+s.prepare()
+o.lib.orc_prepare_turb(o.ptr)
+s.step()
+o.lib.orc_step_turb(o.ptr)
+s.pull()
+for f in ("u", "v", "w", "p", "f_x", "f_y", "f_z"):
+    e = rel(s.array(f), o.array(f))
+    assert e < 1e-9, ("precursor", f, e)
+s.close()
+print("emulated step precursor ok", flush=True)
 """
 
 
@@ -86,4 +103,4 @@ def test_emulated_step_matches_oracle():
     r = subprocess.run([sys.executable, "-c", code], env=env,
                        capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
-    assert r.stdout.count("ok") == 2
+    assert r.stdout.count("ok") == 3
