@@ -111,6 +111,10 @@ int bb_vtk_out(const char *dir, int step);    /* all of them + out.pvd */
 int bb_vtk_turb_init(const char *dir);        /* empty out_turb.pvd */
 int bb_vtk_turb_grid(const char *dir, int step); /* out_turb_N.pvtr, _N_0.vtr */
 int bb_vtk_turb_out(const char *dir, int step);  /* both + out_turb.pvd */
+/* the debugging output with the ghost cells (src/vtk.c:612-933) */
+int bb_vtk_ghost_init(const char *dir);        /* empty out_ghost.pvd */
+int bb_vtk_ghost_grid(const char *dir, int step); /* out_ghost_N.pvtr, _N_0.vtr */
+int bb_vtk_ghost_out(const char *dir, int step);  /* + out_N.vtp, out_ghost.pvd */
 /* the reference's names, on ROOT_DIR/output; they exit on error */
 void init_VTK(void);
 void out_VTK(void);
@@ -120,6 +124,9 @@ void quadnodes_out_VTK(void);
 void init_VTK_turb(void);
 void out_VTK_turb(void);
 void dom_out_VTK_turb(void);
+void init_VTK_ghost(void);
+void out_VTK_ghost(void);
+void dom_out_VTK_ghost(void);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,9 @@
  *   out_turb.pvd, out_turb_N.pvtr, out_turb_N_0.vtr
  *                       the precursor domain's output (src/vtk.c:324-611):
  *                       one DataSet per output, cell data p and vel only
+ *   out_ghost.pvd, out_ghost_N.pvtr, out_ghost_N_0.vtr
+ *                       the debugging output with the ghost cells
+ *                       (src/vtk.c:612-933), plus out_N.vtp
  * The images follow the reference's order: for each particle, the single
  * face images (x low, x high, y low, y high, z low, z high), the edge
  * images of the xy, zy and xz pairs, the corner images, then the particle
@@ -262,6 +265,146 @@ int bb_vtk_turb_grid(const char *dir, int step)
 }
 
 /* ---------------------------------------------------------------------
+ * The debugging grid output with the ghost cells (src/vtk.c:612-933):
+ * out_ghost_N.pvtr + out_ghost_N_0.vtr over all (xn+2)(yn+2)(zn+2) cells,
+ * with p, phase, phase_shell, and the face values of u, u* and the face
+ * flags averaged to the cell centres; the coordinates start one cell
+ * below the domain.  The piece extent is the truncated cell count plus
+ * the two ghost layers.
+ * ------------------------------------------------------------------- */
+/* cells [i0, i1) x [j0, j1) x [k0, k1) of the ghosted grid */
+typedef struct { int i0, i1, j0, j1, k0, k1; } cell_range;
+
+static void centre3(FILE *f, const real *x, const real *y, const real *z,
+  cell_range r)
+{
+  const grid_info *gx = &Dom.Gfx, *gy = &Dom.Gfy, *gz = &Dom.Gfz;
+  for(int k = r.k0; k < r.k1; k++)
+    for(int j = r.j0; j < r.j1; j++)
+      for(int i = r.i0; i < r.i1; i++) {
+        const int cx = i + j * gx->s1b + k * gx->s2b;
+        const int cy = i + j * gy->s1b + k * gy->s2b;
+        const int cz = i + j * gz->s1b + k * gz->s2b;
+        fprintf(f, "%e %e %e ", 0.5 * (x[cx] + x[cx + 1]),
+          0.5 * (y[cy] + y[cy + gy->s1b]), 0.5 * (z[cz] + z[cz + gz->s2b]));
+      }
+}
+
+static void centre3i(FILE *f, const int *x, const int *y, const int *z,
+  cell_range r)
+{
+  const grid_info *gx = &Dom.Gfx, *gy = &Dom.Gfy, *gz = &Dom.Gfz;
+  for(int k = r.k0; k < r.k1; k++)
+    for(int j = r.j0; j < r.j1; j++)
+      for(int i = r.i0; i < r.i1; i++) {
+        const int cx = i + j * gx->s1b + k * gx->s2b;
+        const int cy = i + j * gy->s1b + k * gy->s2b;
+        const int cz = i + j * gz->s1b + k * gz->s2b;
+        fprintf(f, "%lf %lf %lf ", 0.5 * (x[cx] + x[cx + 1]),
+          0.5 * (y[cy] + y[cy + gy->s1b]), 0.5 * (z[cz] + z[cz + gz->s2b]));
+      }
+}
+
+int bb_vtk_ghost_grid(const char *dir, int step)
+{
+  const grid_info *gc = &Dom.Gcc;
+  const dom_struct *sd = &dom[0];
+  const int xl = (int) ((sd->xe - sd->xs) / Dom.dx + 2 * DOM_BUF);
+  const int xs0 = (int) ((sd->xs - Dom.xs) / Dom.dx);
+  const int yl = (int) ((sd->ye - sd->ys) / Dom.dy + 2 * DOM_BUF);
+  const int ys0 = (int) ((sd->ys - Dom.ys) / Dom.dy);
+  const int zl = (int) ((sd->ze - sd->zs) / Dom.dz + 2 * DOM_BUF);
+  const int zs0 = (int) ((sd->zs - Dom.zs) / Dom.dz);
+  const int wx = Dom.xn + 2 * DOM_BUF, wy = Dom.yn + 2 * DOM_BUF;
+  const int wz = Dom.zn + 2 * DOM_BUF;
+  const char *vectors[3] = {"vel", "vel_star", "flag"};
+  FILE *f = open_out(dir, "out_ghost_%d.pvtr", step);
+  if(f == NULL) return -1;
+  fprintf(f, "<VTKFile type=\"PRectilinearGrid\">\n");
+  fprintf(f, "<PRectilinearGrid WholeExtent=\"0 %d 0 %d 0 %d\" "
+    "GhostLevel=\"0\">\n", wx, wy, wz);
+  fprintf(f, "<PCellData Scalars=\"p phase phase_shell\" "
+    "Vectors=\"vel vel_star flag\">\n");
+  fprintf(f, "<PDataArray type=\"Float32\" Name=\"p\"/>\n");
+  fprintf(f, "<PDataArray type=\"Int32\" Name=\"phase\"/>\n");
+  fprintf(f, "<PDataArray type=\"Int32\" Name=\"phase_shell\"/>\n");
+  for(int q = 0; q < 3; q++)
+    fprintf(f, "<PDataArray type=\"Float32\" Name=\"%s\" "
+      "NumberOfComponents=\"3\"/>\n", vectors[q]);
+  fprintf(f, "</PCellData>\n<PCoordinates>\n");
+  for(int a = 0; a < 3; a++)
+    fprintf(f, "<PDataArray type=\"Float32\" Name=\"%c\"/>\n", "xyz"[a]);
+  fprintf(f, "</PCoordinates>\n");
+  fprintf(f, "<Piece Extent=\"%d %d %d %d %d %d\" "
+    "Source=\"out_ghost_%d_0.vtr\"/>\n", xs0, xs0 + xl, ys0, ys0 + yl, zs0,
+    zs0 + zl, step);
+  fprintf(f, "</PRectilinearGrid>\n</VTKFile>\n");
+  fclose(f);
+
+  f = open_out(dir, "out_ghost_%d_0.vtr", step);
+  if(f == NULL) return -1;
+  fprintf(f, "<VTKFile type=\"RectilinearGrid\">\n");
+  fprintf(f, "<RectilinearGrid WholeExtent=\"0 %d 0 %d 0 %d\" "
+    "GhostLevel=\"0\">\n", wx, wy, wz);
+  fprintf(f, "<Piece Extent=\"%d %d %d %d %d %d\">\n", xs0, xs0 + xl, ys0,
+    ys0 + yl, zs0, zs0 + zl);
+  fprintf(f, "<CellData Scalars=\"p phase phase_shell\" "
+    "Vectors=\"vel vel_star flag\">\n");
+  /* the cell loops cover the piece; with one subdomain that is every cell
+   * of the ghosted grid, in k, j, i order */
+  const int k0 = zs0 + gc->ksb, k1 = zs0 + zl + gc->ksb;
+  const int j0 = ys0 + gc->jsb, j1 = ys0 + yl + gc->jsb;
+  const int i0 = xs0 + gc->isb, i1 = xs0 + xl + gc->isb;
+  if(k1 > gc->knb || j1 > gc->jnb || i1 > gc->inb) {
+    fprintf(stderr, "out_ghost: piece extent beyond the ghosted grid\n");
+    fclose(f);
+    return -1;
+  }
+  fprintf(f, "<DataArray type=\"Float32\" Name=\"p\">\n");
+  for(int k = k0; k < k1; k++)
+    for(int j = j0; j < j1; j++)
+      for(int i = i0; i < i1; i++)
+        fprintf(f, "%e ", p[i + j * gc->s1b + k * gc->s2b]);
+  fprintf(f, "\n</DataArray>\n");
+  const int *iv[2] = {phase, phase_shell};
+  const char *inames[2] = {"phase", "phase_shell"};
+  for(int q = 0; q < 2; q++) {
+    fprintf(f, "<DataArray type=\"Int32\" Name=\"%s\">\n", inames[q]);
+    for(int k = k0; k < k1; k++)
+      for(int j = j0; j < j1; j++)
+        for(int i = i0; i < i1; i++)
+          fprintf(f, "%d ", iv[q][i + j * gc->s1b + k * gc->s2b]);
+    fprintf(f, "\n</DataArray>\n");
+  }
+  fprintf(f, "<DataArray type=\"Float32\" Name=\"vel\" NumberOfComponents=\"3\""
+    " format=\"ascii\">\n");
+  const cell_range rg = {i0, i1, j0, j1, k0, k1};
+  centre3(f, u, v, w, rg);
+  fprintf(f, "\n</DataArray>\n");
+  fprintf(f, "<DataArray type=\"Float32\" Name=\"vel_star\" "
+    "NumberOfComponents=\"3\" format=\"ascii\">\n");
+  centre3(f, u_star, v_star, w_star, rg);
+  fprintf(f, "\n</DataArray>\n");
+  fprintf(f, "<DataArray type=\"Float32\" Name=\"flag\" NumberOfComponents=\"3\""
+    " format=\"ascii\">\n");
+  centre3i(f, flag_u, flag_v, flag_w, rg);
+  fprintf(f, "\n</DataArray>\n</CellData>\n<Coordinates>\n");
+  const real s0[3] = {sd->xs, sd->ys, sd->zs};
+  const real h[3] = {Dom.dx, Dom.dy, Dom.dz};
+  const int n[3] = {xl, yl, zl};
+  for(int a = 0; a < 3; a++) {
+    fprintf(f, "<DataArray type=\"Float32\" Name=\"%c\">\n", "xyz"[a]);
+    for(int q = -1; q < n[a]; q++) fprintf(f, "%lf ", s0[a] + h[a] * q);
+    fprintf(f, "\n</DataArray>\n");
+  }
+  fprintf(f, "</Coordinates>\n</Piece>\n</RectilinearGrid>\n</VTKFile>\n");
+  fclose(f);
+  return 0;
+}
+
+int bb_vtk_ghost_init(const char *dir) { return pvd_init(dir, "out_ghost.pvd"); }
+
+/* ---------------------------------------------------------------------
  * Particles: out_N.vtp
  * ------------------------------------------------------------------- */
 static void vec_array(FILE *f, const char *name, const part_struct *pv, int n,
@@ -441,6 +584,15 @@ int bb_vtk_turb_out(const char *dir, int step)
   return bb_vtk_turb_grid(dir, step);
 }
 
+/* one debugging output: the ghosted grid and the particles */
+int bb_vtk_ghost_out(const char *dir, int step)
+{
+  static const char *const kind[2] = {"out_ghost_%d.pvtr", "out_%d.vtp"};
+  if(pvd_append(dir, "out_ghost.pvd", step, kind, 2)) return -1;
+  if(bb_vtk_ghost_grid(dir, step)) return -1;
+  return bb_vtk_parts(dir, step);
+}
+
 /* ---- the reference's names, on ROOT_DIR/output, exiting on error ---- */
 static void output_dir(char *buf, size_t len)
 {
@@ -501,4 +653,25 @@ void dom_out_VTK_turb(void)
   char d[2 * FILE_NAME_SIZE];
   output_dir(d, sizeof(d));
   if(bb_vtk_turb_grid(d, rec_prec_stepnum_out)) exit(EXIT_FAILURE);
+}
+
+void init_VTK_ghost(void)
+{
+  char d[2 * FILE_NAME_SIZE];
+  output_dir(d, sizeof(d));
+  if(bb_vtk_ghost_init(d)) exit(EXIT_FAILURE);
+}
+
+void out_VTK_ghost(void)
+{
+  char d[2 * FILE_NAME_SIZE];
+  output_dir(d, sizeof(d));
+  if(bb_vtk_ghost_out(d, rec_paraview_stepnum_out)) exit(EXIT_FAILURE);
+}
+
+void dom_out_VTK_ghost(void)
+{
+  char d[2 * FILE_NAME_SIZE];
+  output_dir(d, sizeof(d));
+  if(bb_vtk_ghost_grid(d, rec_paraview_stepnum_out)) exit(EXIT_FAILURE);
 }

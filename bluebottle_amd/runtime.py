@@ -91,6 +91,9 @@ HOST_FUNCS = {
     "bb_vtk_turb_init": (C.c_int, [C.c_char_p]),
     "bb_vtk_turb_grid": (C.c_int, [C.c_char_p, C.c_int]),
     "bb_vtk_turb_out": (C.c_int, [C.c_char_p, C.c_int]),
+    "bb_vtk_ghost_init": (C.c_int, [C.c_char_p]),
+    "bb_vtk_ghost_grid": (C.c_int, [C.c_char_p, C.c_int]),
+    "bb_vtk_ghost_out": (C.c_int, [C.c_char_p, C.c_int]),
     "bb_read_turb_config": (C.c_int, [C.c_char_p]),
     "bb_out_restart_turb": (C.c_int, [C.c_char_p]),
     "bb_in_restart_turb": (C.c_int, [C.c_char_p]),

@@ -255,3 +255,69 @@ def test_precursor_domain_files(tmp_path):
                                  ("5.000000e-01", "0", "out_turb_4.pvtr")]
     finally:
         s.close()
+
+
+def test_ghost_cell_output(tmp_path):
+    """out_ghost_N_0.vtr (src/vtk.c:612-933) covers every cell of the
+    ghosted grid in k, j, i order: p, phase, phase_shell, the face values of
+    u, u* and the face flags averaged to the cell centres (ghost faces
+    included), coordinates from one cell below the domain; out_ghost.pvd
+    gains the grid and the particle file."""
+    case = sim.Case(n=(5, 4, 3), hi=(1.25, 1.0, 0.75), nu=0.01,
+                    init="taylor_green", noise=1e-3, seed=2,
+                    particles=[sim.Particle(x=0.6, y=0.5, z=0.4, r=0.2)])
+    s = sim.Simulation(case)
+    try:
+        rng = np.random.default_rng(5)
+        for nm in ("p", "u", "v", "w", "u_star", "v_star", "w_star"):
+            a = s.array(nm)
+            a[:] = rng.standard_normal(a.size)
+        for nm in ("phase", "phase_shell", "flag_u", "flag_v", "flag_w"):
+            a = s.array(nm)
+            a[:] = rng.integers(-1, 3, a.size)
+        d = str(tmp_path).encode()
+        runtime.set_scalar("ttime", 0.125)
+        assert s.host.bb_vtk_ghost_init(d) == 0
+        assert s.host.bb_vtk_ghost_out(d, 2) == 0
+        root = ET.parse(tmp_path / "out_ghost_2_0.vtr").getroot()
+        assert root.find("RectilinearGrid").get("WholeExtent") == "0 7 0 6 0 5"
+        assert root.find("RectilinearGrid/Piece").get("Extent") == "0 7 0 6 0 5"
+        A = _arrays(root)
+        assert [float(x) for x in A["p"]] == [float("%e" % x)
+                                              for x in s.array("p")]
+        assert [int(x) for x in A["phase_shell"]] == list(s.array("phase_shell"))
+        D = s.dom
+        gc = D.Gcc
+        nc = gc.knb * gc.jnb * gc.inb
+
+        def centres(xn, yn, zn):
+            x = s.field3(xn)
+            y = s.field3(yn)
+            z = s.field3(zn)
+            return (0.5 * (x[:, :, :-1] + x[:, :, 1:]),
+                    0.5 * (y[:, :-1, :] + y[:, 1:, :]),
+                    0.5 * (z[:-1] + z[1:]))
+
+        for name, comps, fmt in (("vel", ("u", "v", "w"), "%e"),
+                                 ("vel_star", ("u_star", "v_star", "w_star"),
+                                  "%e")):
+            got = np.array([float(x) for x in A[name]]).reshape(nc, 3)
+            for c, ref in enumerate(centres(*comps)):
+                assert ref.shape == (gc.knb, gc.jnb, gc.inb)
+                assert np.array_equal(got[:, c], [float(fmt % x)
+                                                  for x in ref.ravel()])
+        fl = np.array([float(x) for x in A["flag"]]).reshape(nc, 3)
+        fu = s.array("flag_u").reshape(gc.knb, gc.jnb, gc.inb + 1)
+        assert np.array_equal(fl[:, 0], (0.5 * (fu[:, :, :-1] + fu[:, :, 1:])).ravel())
+        assert np.allclose([float(x) for x in A["x"]], -0.25 + 0.25 * np.arange(8))
+        assert np.allclose([float(x) for x in A["z"]], -0.25 + 0.25 * np.arange(6))
+        ds = ET.parse(tmp_path / "out_ghost.pvd").getroot().findall(
+            "Collection/DataSet")
+        assert [(x.get("part"), x.get("file")) for x in ds] == [
+            ("0", "out_ghost_2.pvtr"), ("1", "out_2.vtp")]
+        assert (tmp_path / "out_2.vtp").exists()
+        hdr = ET.parse(tmp_path / "out_ghost_2.pvtr").getroot()
+        assert [x.get("Name") for x in hdr.iter("PDataArray")] == [
+            "p", "phase", "phase_shell", "vel", "vel_star", "flag", "x", "y", "z"]
+    finally:
+        s.close()
