@@ -336,3 +336,33 @@ def test_recorder_init_files(tmp_path):
                                                  "niter", "resid")
     assert (tmp_path / "record" / "lamb.rec").read_text() == ""
     buf.in_dll(host, "bb_root_dir").value = b"."
+
+
+def test_reference_lid_driven_cavity_config(tmp_path):
+    """flow.config-lid_driven_cavity: Dirichlet walls on four faces with the
+    lid (north) moving at u = 10, Neumann pressure there, periodic z; like
+    the sedimentation file it has no *_tDelay lines, so the g lines are
+    lost to the "g_tDelay" attempt and the PID gains after them do not
+    parse either: those keep their previous values."""
+    runtime.set_scalar("Kp", 0.25)
+    g = runtime.gvar("g", abi.g_struct)
+    g.ym = 4.0
+    _read_flow("flow.config-lid_driven_cavity", tmp_path)
+    D = runtime.gvar("Dom", abi.dom_struct)
+    assert (D.xs, D.xe, D.xn) == (-4.0, 4.0, 64)
+    b = runtime.bc()
+    for f in "WESN":
+        assert getattr(b, "p" + f) == abi.NEUMANN
+        for c in "uvw":
+            assert getattr(b, c + f) == abi.DIRICHLET
+    assert (b.pB, b.uT, b.wB) == (abi.PERIODIC,) * 3
+    assert (b.uNDm, b.uNDa, b.uND) == (10.0, 0.0, 0.0)
+    assert b.uSDm == 0.0 and b.vNDm == 0.0
+    assert runtime.get_scalar("nu") == 1.0
+    assert runtime.get_scalar("lamb_max_iter", "i") == 2000
+    assert runtime.get_scalar("init_cond", "i") == abi.SHEAR
+    assert runtime.get_scalar("out_plane", "i") == abi.NORTH
+    gp = runtime.gvar("gradP", abi.gradP_struct)
+    assert (gp.xm, gp.xa, gp.zm) == (0.0, 0.0, 0.0)
+    assert g.ym == 4.0
+    assert runtime.get_scalar("Kp") == 0.25
