@@ -304,6 +304,80 @@ def write_case(case, root):
         f.write("\n".join(P) + "\n")
 
 
+def write_turb_config(case, root):
+    """Write root/input/turb.config, the precursor domain's file
+    (src/domain.c:732-1287): the flow.config grammar without the Lamb
+    parameters, vel_tDelay, the initial condition and the PID gains, and
+    turbA as its only driving condition."""
+    import os
+    os.makedirs(os.path.join(root, "input"), exist_ok=True)
+    c = case
+    ext = ["(%ss, %se, %sn) %s %s %d" % (nm, nm, nm, _g(c.lo[ax]), _g(c.hi[ax]),
+                                         c.n[ax])
+           for ax, nm in enumerate("XYZ")]
+    L = ["DOMAIN"] + ext + ["", "GPU DOMAIN DECOMPOSITION", "DEV RANGE 0 0",
+                            "n 1"] + ext
+    L += ["E -1 W -1 N -1 S -1 T -1 B -1", "", "PHYSICAL PARAMETERS",
+          "rho_f " + _g(c.rho_f), "nu " + _g(c.nu), "", "SIMULATION PARAMETERS",
+          "duration " + _g(c.duration), "CFL " + _g(c.CFL),
+          "pp_max_iter %d" % c.pp_max_iter,
+          "pp_residual " + _g(c.pp_residual), "", "BOUNDARY CONDITIONS"]
+    for title, comp, spec in (("PRESSURE", "p", c.bc_p),
+                              ("X-VELOCITY", "u", c.bc_u),
+                              ("Y-VELOCITY", "v", c.bc_v),
+                              ("Z-VELOCITY", "w", c.bc_w)):
+        L.append(title)
+        for f, face in enumerate(_FACES):
+            t, val = spec[f]
+            if t == abi.PRECURSOR:
+                raise ValueError("the precursor domain has no PRECURSOR faces")
+            line = "bc.%s%s %s" % (comp, face, _BC_NAME[t])
+            if comp == "p" and t == abi.NEUMANN:
+                line += " 0"
+            elif comp != "p" and t == abi.DIRICHLET:
+                line += " %s 0" % _g(val)
+            L.append(line)
+    L += ["", "SOLVABILITY ENFORCEMENT PLANE",
+          "out_plane " + _PLANE_NAME[c.out_plane], "",
+          "SIMULATION DRIVING CONDITIONS", "turbA " + _g(c.turbA)]
+    with open(os.path.join(root, "input", "turb.config"), "w") as f:
+        f.write("\n".join(L) + "\n")
+
+
+def read_turb_case(root, seed=1):
+    """A precursor-domain Case from root/input/turb.config, parsed by the
+    driver library's reader; its random initial field is drawn from
+    `seed` (the reference seeds from the clock)."""
+    import os
+    from . import runtime
+    host, _ = runtime.load()
+    path = os.path.join(root, "input", "turb.config").encode()
+    if host.bb_read_turb_config(path) != 0:
+        raise ValueError("could not read %s" % path.decode())
+    D = runtime.gvar("Dom", abi.dom_struct)
+    b = runtime.bc()
+
+    def spec(comp):
+        out = []
+        for face in _FACES:
+            t = getattr(b, comp + face)
+            v = getattr(b, comp + face + "Dm") if comp != "p" and \
+                t == abi.DIRICHLET else 0.0
+            out.append((t, v))
+        return tuple(out)
+
+    gs = runtime.get_scalar
+    return Case(n=(D.xn, D.yn, D.zn), lo=(D.xs, D.ys, D.zs),
+                hi=(D.xe, D.ye, D.ze), rho_f=gs("rho_f"), nu=gs("nu"),
+                CFL=gs("CFL"), pp_max_iter=gs("pp_max_iter", "i"),
+                pp_residual=gs("pp_residual"), lamb_max_iter=0,
+                lamb_residual=0.0, lamb_relax=0.0, lamb_cut=0.0,
+                out_plane=gs("out_plane", "i"), bc_p=spec("p"),
+                bc_u=spec("u"), bc_v=spec("v"), bc_w=spec("w"),
+                duration=gs("duration"), init="quiescent", seed=seed,
+                precursor=True, turbA=gs("turbA"))
+
+
 def read_case(root, noise=1e-3, seed=1):
     """A Case from root/input/flow.config and root/input/part.config,
     parsed by the driver library's readers (the reference's grammar)."""

@@ -421,3 +421,32 @@ def test_driver_sequence_with_reference_files(tmp_path):
     finally:
         host.bb_host_free()
         root.in_dll(host, "bb_root_dir").value = old
+
+
+def test_turb_config_round_trip_and_reference_file(tmp_path):
+    """configs.write_turb_config writes the reference's grammar: a precursor
+    case read back from it is the same case, and the reference's own
+    turb.config reads into a precursor case with its values."""
+    from bluebottle_amd import configs
+    D, N, P = abi.DIRICHLET, abi.NEUMANN, abi.PERIODIC
+    case = sim.Case(n=(10, 8, 6), lo=(-1.0, 0.0, 0.5), hi=(1.5, 2.0, 2.0),
+                    nu=0.125, CFL=0.4, pp_max_iter=321, pp_residual=1e-9,
+                    duration=2.5, out_plane=abi.HOMOGENEOUS,
+                    bc_p=((P, 0.), (P, 0.), (N, 0.), (N, 0.), (P, 0.), (P, 0.)),
+                    bc_u=((P, 0.), (P, 0.), (D, -0.25), (D, 0.75), (P, 0.), (P, 0.)),
+                    bc_v=((P, 0.), (P, 0.), (D, 0.), (D, 0.), (P, 0.), (P, 0.)),
+                    bc_w=((P, 0.), (P, 0.), (N, 0.), (N, 0.), (P, 0.), (P, 0.)),
+                    precursor=True, turbA=1.75, init="quiescent", seed=9,
+                    lamb_max_iter=0, lamb_residual=0.0, lamb_relax=0.0,
+                    lamb_cut=0.0)
+    configs.write_turb_config(case, str(tmp_path))
+    back = configs.read_turb_case(str(tmp_path), seed=9)
+    assert back == case
+    ref = tmp_path / "ref"
+    (ref / "input").mkdir(parents=True)
+    (ref / "input" / "turb.config").write_text(
+        open(os.path.join(REF_INPUT, "turb.config")).read())
+    c = configs.read_turb_case(str(ref))
+    assert c.precursor and c.turbA == 3.0 and c.n == (64, 64, 64)
+    assert (c.lo, c.hi) == ((-4.0,) * 3, (4.0,) * 3)
+    assert c.out_plane == abi.TOP and c.nu == 0.5
