@@ -314,6 +314,9 @@ class Exec:
         self.block = block
         self.cache = {}
         self.log = log
+        # instructions each finished wavefront issued (a latency proxy: the
+        # longest is the launch's critical path in issue slots)
+        self.wave_instructions = []
 
     # ---- operand helpers ----
     def sreg(self, w, name, n=1):
@@ -562,12 +565,15 @@ class Exec:
             addr, op, rest, tgt = k.insts[w.pc]
             w.pc += 1
             steps += 1
+            w.issued = getattr(w, "issued", 0) + 1
             if steps > 50_000_000:
                 raise RuntimeError("%s: runaway loop" % k.name)
             if op == "s_barrier":
                 w.at_barrier = True
                 return
             self.execute(w, op, rest, tgt, (k.name, addr, gid))
+            if w.done:
+                self.wave_instructions.append(w.issued)
 
     def branch(self, w, tgt):
         w.pc = self.k.index[tgt]

@@ -204,6 +204,10 @@ def replay_one(lib, stem, verbose=True):
               "worst fp64 rel %.1e" % (os.path.basename(stem), status,
                                       str(tuple(meta["grid"])), len(mem.violations),
                                       len(mism), worst), flush=True)
+        if os.environ.get("BB_ISA_COUNT") and ex.wave_instructions:
+            print("    wave instructions: longest %d, all waves %d" % (
+                max(ex.wave_instructions), sum(ex.wave_instructions)),
+                flush=True)
         for v in mem.violations[:5]:
             print("    outside:", v, flush=True)
         for x in mism[:5]:
