@@ -44,6 +44,10 @@ LambNorm bb_lamb_norm(void)
   return L;
 }
 
+/* blocks per component of the face-list kernels (k_face_count, k_face_scan,
+ * k_face_write) */
+#define FL_BLOCKS 1024
+
 void bb_ensure_particle_workspace(void)
 {
   bb_ensure_context();
@@ -57,7 +61,9 @@ void bb_ensure_particle_workspace(void)
     BB_CHECK(hipMalloc(&g_bb.face_list, (size_t) 3 * cap * 2 * sizeof(int)));
     g_bb.face_cap = 3 * cap;
   }
-  if(!g_bb.face_count) BB_CHECK(hipMalloc(&g_bb.face_count, 4 * sizeof(int)));
+  /* the three list lengths, then per-block counts and offsets */
+  if(!g_bb.face_count)
+    BB_CHECK(hipMalloc(&g_bb.face_count, (4 + 6 * FL_BLOCKS) * sizeof(int)));
   if(!g_bb.face_count_host)
     BB_CHECK(hipHostMalloc((void **) &g_bb.face_count_host, 4 * sizeof(int),
       hipHostMallocDefault));
@@ -296,50 +302,138 @@ k_face_edge(int *__restrict__ f, int s1, int s2, int n0, int n1, int n2,
   }
 }
 
-/* list of faces that part_BC writes: flag < 1 with a solid neighbour;
- * the transverse range includes the ghost rows, as in the reference.
- * Each wavefront claims the slots of all its listed faces with one atomic
- * (its lowest listing lane adds the ballot's population count); a lane's
- * slot is that base plus the listing lanes below it.  The loop runs the
- * same number of times in every lane of a block, so every lane takes part
- * in each ballot.  The list order is whatever the atomics give, as
- * before: part_BC writes each face independently. */
-__global__ void __launch_bounds__(256)
-k_face_list(BBGeom G, int comp, const int *__restrict__ ph,
-  const int *__restrict__ flag, int *__restrict__ list, int *__restrict__ count,
-  int cap)
+/* Lists of the faces that part_BC writes: flag < 1 with a solid
+ * neighbour; the transverse range includes the ghost rows, as in the
+ * reference.  Appending through one global counter serialises every
+ * wavefront that finds a face on that one address (the compiler already
+ * merges a wavefront's increments into one atomic), so the lists are built
+ * without it: each of FL_BLOCKS blocks owns a contiguous range of a
+ * component's faces, counts its listed faces, a one-block scan turns the
+ * counts into offsets, and each block writes its faces from its offset in
+ * index order (the lists come out sorted by face index). */
+struct FaceGrid { int ni, nj, nk, s1, s2, off_i, off_j, off_k, cs; };
+
+__device__ inline FaceGrid face_grid(const BBGeom &G, int comp)
 {
-  int ni, nj, nk, s1, s2, off_i, off_j, off_k, cs;
-  if(comp == 0) { ni = G.xn + 1; nj = G.yn + 2; nk = G.zn + 2; s1 = G.us1; s2 = G.us2; off_i = 1; off_j = 0; off_k = 0; cs = 1; }
-  else if(comp == 1) { ni = G.xn + 2; nj = G.yn + 1; nk = G.zn + 2; s1 = G.vs1; s2 = G.vs2; off_i = 0; off_j = 1; off_k = 0; cs = G.cs1; }
-  else { ni = G.xn + 2; nj = G.yn + 2; nk = G.zn + 1; s1 = G.ws1; s2 = G.ws2; off_i = 0; off_j = 0; off_k = 1; cs = G.cs2; }
-  const long total = (long) ni * nj * nk;
-  const long stride = (long) gridDim.x * 256;
-  const int lane = threadIdx.x & 63;
+  FaceGrid F;
+  if(comp == 0) {
+    F.ni = G.xn + 1; F.nj = G.yn + 2; F.nk = G.zn + 2; F.s1 = G.us1;
+    F.s2 = G.us2; F.off_i = 1; F.off_j = 0; F.off_k = 0; F.cs = 1;
+  } else if(comp == 1) {
+    F.ni = G.xn + 2; F.nj = G.yn + 1; F.nk = G.zn + 2; F.s1 = G.vs1;
+    F.s2 = G.vs2; F.off_i = 0; F.off_j = 1; F.off_k = 0; F.cs = G.cs1;
+  } else {
+    F.ni = G.xn + 2; F.nj = G.yn + 2; F.nk = G.zn + 1; F.s1 = G.ws1;
+    F.s2 = G.ws2; F.off_i = 0; F.off_j = 0; F.off_k = 1; F.cs = G.cs2;
+  }
+  return F;
+}
+
+/* the particle face q of the component's range touches (or -1), and its
+ * index in the component's array */
+__device__ inline int face_entry(const BBGeom &G, const FaceGrid &F,
+  const int *__restrict__ ph, const int *__restrict__ flag, long q, int *C)
+{
+  const int i = (int) (q % F.ni) + F.off_i;
+  const int j = (int) ((q / F.ni) % F.nj) + F.off_j;
+  const int k = (int) (q / ((long) F.ni * F.nj)) + F.off_k;
+  *C = i + j*F.s1 + k*F.s2;
+  if(flag[*C] >= 1) return -1;
+  const int Cc = i + j*G.cs1 + k*G.cs2;
+  const int p0 = ph[Cc - F.cs], p1 = ph[Cc];
+  return p0 > -1 ? p0 : p1;
+}
+
+/* block b's range [lo, hi) of the component's faces, of nb blocks */
+__device__ inline void face_range(const FaceGrid &F, int b, int nb, long *lo,
+  long *hi)
+{
+  const long total = (long) F.ni * F.nj * F.nk;
+  const long chunk = (total + nb - 1) / nb;
+  *lo = (long) b * chunk;
+  *hi = *lo + chunk < total ? *lo + chunk : total;
+}
+
+/* cnt[comp * FL_BLOCKS + b]: listed faces in block b's range (gridDim.x
+ * blocks per component, at most FL_BLOCKS) */
+__global__ void __launch_bounds__(256)
+k_face_count(BBGeom G, const int *__restrict__ ph, const int *__restrict__ fu,
+  const int *__restrict__ fv, const int *__restrict__ fw, int *__restrict__ cnt)
+{
+  __shared__ double sh[4];
+  const int comp = blockIdx.y;
+  const FaceGrid F = face_grid(G, comp);
+  const int *flag = comp == 0 ? fu : comp == 1 ? fv : fw;
+  long lo, hi;
+  face_range(F, blockIdx.x, gridDim.x, &lo, &hi);
+  int n = 0;
+  for(long q = lo + threadIdx.x; q < hi; q += 256) {
+    int C;
+    n += face_entry(G, F, ph, flag, q, &C) >= 0;
+  }
+  /* exact: the counts are far below 2^53 */
+  const double s = bb_block_sum((double) n, sh);
+  if(threadIdx.x == 0) cnt[comp * FL_BLOCKS + blockIdx.x] = (int) s;
+}
+
+/* per component (one block each): exclusive offsets of the nb block
+ * counts and the list length */
+__global__ void __launch_bounds__(FL_BLOCKS)
+k_face_scan(int nb, const int *__restrict__ cnt, int *__restrict__ off,
+  int *__restrict__ total)
+{
+  __shared__ int a[FL_BLOCKS];
+  const int comp = blockIdx.x, t = threadIdx.x;
+  const int v = t < nb ? cnt[comp * FL_BLOCKS + t] : 0;
+  a[t] = v;
+  __syncthreads();
+  for(int d = 1; d < FL_BLOCKS; d <<= 1) {
+    const int x = t >= d ? a[t - d] : 0;
+    __syncthreads();
+    a[t] += x;
+    __syncthreads();
+  }
+  off[comp * FL_BLOCKS + t] = a[t] - v;
+  if(t == FL_BLOCKS - 1) total[comp] = a[t];
+}
+
+/* each block writes its listed faces from its offset on, in index order:
+ * per 256-face step, a wavefront's ballot gives each lane its rank and the
+ * four wavefront counts (in LDS) the wavefronts' order; the loop runs the
+ * same number of times in every lane of the block */
+__global__ void __launch_bounds__(256)
+k_face_write(BBGeom G, const int *__restrict__ ph, const int *__restrict__ fu,
+  const int *__restrict__ fv, const int *__restrict__ fw,
+  const int *__restrict__ off, int *__restrict__ lists, int cap)
+{
+  __shared__ int wsum[4];
+  const int comp = blockIdx.y;
+  const FaceGrid F = face_grid(G, comp);
+  const int *flag = comp == 0 ? fu : comp == 1 ? fv : fw;
+  int *list = lists + (size_t) 2 * cap * comp;
+  long lo, hi;
+  face_range(F, blockIdx.x, gridDim.x, &lo, &hi);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const unsigned long long below = (1ull << lane) - 1ull;
-  for(long base = (long) blockIdx.x * 256; base < total; base += stride) {
-    const long q = base + threadIdx.x;
+  int base = off[comp * FL_BLOCKS + blockIdx.x];
+  for(long q0 = lo; q0 < hi; q0 += 256) {
+    const long q = q0 + threadIdx.x;
     int C = 0, P = -1;
-    if(q < total) {
-      const int i = (int) (q % ni) + off_i;
-      const int j = (int) ((q / ni) % nj) + off_j;
-      const int k = (int) (q / ((long) ni * nj)) + off_k;
-      C = i + j*s1 + k*s2;
-      if(flag[C] < 1) {
-        const int Cc = i + j*G.cs1 + k*G.cs2;
-        const int p0 = ph[Cc - cs], p1 = ph[Cc];
-        P = p0 > -1 ? p0 : p1;
-      }
-    }
+    if(q < hi) P = face_entry(G, F, ph, flag, q, &C);
     const unsigned long long mask = __ballot(P >= 0);
-    const int leader = mask ? __ffsll((unsigned long long) mask) - 1 : 0;
-    int slot = 0;
-    if(mask && lane == leader) slot = atomicAdd(count, __popcll(mask));
-    slot = __shfl(slot, leader, 64);
+    if(lane == 0) wsum[wave] = __popcll(mask);
+    __syncthreads();
+    int before = 0, all = 0;
+    for(int w = 0; w < 4; w++) {
+      if(w < wave) before += wsum[w];
+      all += wsum[w];
+    }
     if(P >= 0) {
-      slot += __popcll(mask & below);
+      const int slot = base + before + __popcll(mask & below);
       if(slot < cap) { list[2*slot] = C; list[2*slot+1] = P; }
     }
+    base += all;
+    __syncthreads();
   }
 }
 
@@ -351,13 +445,19 @@ static void build_face_lists(void)
   BBGeom G = bb_geom();
   bb_ensure_particle_workspace();
   int cap = g_bb.face_cap / 3;
-  BB_CHECK(hipMemsetAsync(g_bb.face_count, 0, 4 * sizeof(int), g_bb.stream));
-  int *fl[3] = {_flag_u[0], _flag_v[0], _flag_w[0]};
-  for(int c = 0; c < 3; c++) {
-    k_face_list<<<2048, 256, 0, g_bb.stream>>>(G, c, _phase[0], fl[c],
-      g_bb.face_list + (size_t) 2 * cap * c, g_bb.face_count + c, cap);
-    BB_CHECK_LAUNCH();
-  }
+  /* face_count: the three lengths, then the block counts and offsets */
+  int *cnt = g_bb.face_count + 4, *off = cnt + 3 * FL_BLOCKS;
+  /* one block per 256 faces of the largest component, at most FL_BLOCKS */
+  const long nmax = (long) (G.xn + 2) * (G.yn + 2) * (G.zn + 2);
+  const int nb = (int) (nmax / 256 + 1 < FL_BLOCKS ? nmax / 256 + 1 : FL_BLOCKS);
+  k_face_count<<<dim3(nb, 3), 256, 0, g_bb.stream>>>(G, _phase[0],
+    _flag_u[0], _flag_v[0], _flag_w[0], cnt);
+  BB_CHECK_LAUNCH();
+  k_face_scan<<<3, FL_BLOCKS, 0, g_bb.stream>>>(nb, cnt, off, g_bb.face_count);
+  BB_CHECK_LAUNCH();
+  k_face_write<<<dim3(nb, 3), 256, 0, g_bb.stream>>>(G, _phase[0],
+    _flag_u[0], _flag_v[0], _flag_w[0], off, g_bb.face_list, cap);
+  BB_CHECK_LAUNCH();
   BB_CHECK(hipMemcpyAsync(g_bb.face_count_host, g_bb.face_count, 3 * sizeof(int),
     hipMemcpyDeviceToHost, g_bb.stream));
   BB_CHECK(hipStreamSynchronize(g_bb.stream));

@@ -1400,6 +1400,9 @@ class Exec:
         if not m:
             raise NotImplementedError(op)
         kind, t = m.groups()
+        if kind == "atomic":
+            # memory-side atomic operations: one per active lane
+            self.atomic_lanes = getattr(self, "atomic_lanes", 0) + int(act.sum())
         if kind == "load":
             n = sizes[t]
             addr = self._gaddr(w, ops, mods, 1, 2)

@@ -102,17 +102,11 @@ def hidden_args(k, grid, block, buf):
         buf[off:off + size] = int(val).to_bytes(size, "little")
 
 
-# kernels that append 8-byte records through atomics: the record order
-# depends on the order the wavefronts run in, so a list buffer is compared
-# as a multiset of records
-ORDER_FREE = ("k_face_list",)
-
-
-def compare(before, after_emu, after_isa, tol=1e-12, order_free=False):
+def compare(before, after_emu, after_isa, tol=1e-12):
     """Worst FP64 relative difference and list of non-FP mismatches; `tol`
     is the FP64 tolerance (FP32 kernels pass a looser one).  Differences
     where either side reads as a subnormal double are integer data and
-    must match exactly (as sorted 8-byte records when `order_free`)."""
+    must match exactly."""
     bad = []
     worst = 0.0
     if after_emu == after_isa:
@@ -129,10 +123,7 @@ def compare(before, after_emu, after_isa, tol=1e-12, order_free=False):
             sub = ((fa[idx] != 0) & (np.abs(fa[idx]) < tiny)) | \
                 ((fb[idx] != 0) & (np.abs(fb[idx]) < tiny))
         if np.any(sub):
-            ra = np.sort(np.frombuffer(after_emu, dtype=np.int64))
-            rb = np.sort(np.frombuffer(after_isa, dtype=np.int64))
-            if not (order_free and np.array_equal(ra, rb)):
-                bad.append(("int", int(diff[0]), len(diff), 0.0))
+            bad.append(("int", int(diff[0]), len(diff), 0.0))
             return worst, bad
         with np.errstate(all="ignore"):
             sc = np.max(np.abs(fa[np.isfinite(fa)])) if np.any(np.isfinite(fa)) else 1.0
@@ -191,10 +182,9 @@ def replay_one(lib, stem, verbose=True):
     # kernels instantiated for FP32 vectors: FP32 rounding differences
     # between the two builds reach the FP64 dot partials too
     tol = 1e-6 if "IfE" in k.name or "Pf" in k.name else 1e-12
-    order_free = meta["kernel"] in ORDER_FREE
     for base, size, p, b in bufs:
         w, bad = compare(raw_before[p:p + size], raw_after[p:p + size], bytes(b),
-                         tol, order_free)
+                         tol)
         worst = max(worst, w)
         for x in bad:
             mism.append((hex(base), size) + x)
@@ -205,9 +195,10 @@ def replay_one(lib, stem, verbose=True):
                                       str(tuple(meta["grid"])), len(mem.violations),
                                       len(mism), worst), flush=True)
         if os.environ.get("BB_ISA_COUNT") and ex.wave_instructions:
-            print("    wave instructions: longest %d, all waves %d" % (
-                max(ex.wave_instructions), sum(ex.wave_instructions)),
-                flush=True)
+            print("    wave instructions: longest %d, all waves %d; global "
+                  "atomic lane operations %d" % (
+                      max(ex.wave_instructions), sum(ex.wave_instructions),
+                      getattr(ex, "atomic_lanes", 0)), flush=True)
         for v in mem.violations[:5]:
             print("    outside:", v, flush=True)
         for x in mism[:5]:

@@ -127,8 +127,8 @@ def test_precursor_domain_kernels_machine_code_replays_clean(tmp_path):
 
 
 def test_particle_list_and_bin_kernels_machine_code_replays_clean(tmp_path):
-    """The wave-aggregated face lists (k_face_list: one atomic per
-    wavefront, the lists compared as multisets of records), the bin
+    """The face lists (k_face_count, k_face_scan, k_face_write: per-block
+    counts, offsets, and each block's faces written in index order), the bin
     placement (k_bin_fill's slots, k_bin_scatter, k_bin_place's rank within
     the bin) and the wavefront-wide k_lamb_err, on three spheres with a pair
     0.3 r apart."""
@@ -142,14 +142,15 @@ def test_particle_list_and_bin_kernels_machine_code_replays_clean(tmp_path):
     cap = tmp_path / "cap"
     env = dict(os.environ)
     env.update(LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0",
-               BB_EMU_ISA_KERNELS="k_face_list,k_bin_fill,k_bin_scatter,"
-               "k_bin_place,k_lamb_err", BB_EMU_ISA_MAX="4")
+               BB_EMU_ISA_KERNELS="k_face_count,k_face_scan,k_face_write,"
+               "k_bin_fill,k_bin_scatter,k_bin_place,k_lamb_err",
+               BB_EMU_ISA_MAX="4")
     r = subprocess.run([sys.executable, os.path.join(ISA, "capture.py"),
                         "particle_kernels", str(cap)], env=env,
                        capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     launches = glob.glob(str(cap / "*.json"))
-    # three face lists, one bin pass of three kernels, one k_lamb_err
+    # the face lists' three kernels, one bin pass of three, one k_lamb_err
     assert len(launches) == 7, len(launches)
     env = dict(os.environ)
     env["BB_ISA_OBJDIR"] = str(tmp_path / "co")
