@@ -464,41 +464,16 @@ static real p_at(long idx)
   return idx < (long) Dom.Gcc.s3b ? p[idx] : 0.;
 }
 
-/* src/domain.c:2856-3093 after the grids and arrays exist (bb_host_domain):
- * uniform random velocities of amplitude 3 turbA l (l the mean side
- * length, the integral scale turbl), the reference's one-sweep divergence
- * correction and mean removal, zeroed inflow planes, and the first dt
- * from the viscous limit alone.  The generator is seeded with `seed`
- * (the reference uses the wall clock).  As there, the correction of v
- * indexes the cell divergence with v's own strides; values that would lie
- * past the cell array (zn > yn + 2) are taken as 0, where the reference
- * reads beyond it. */
-int bb_host_init_turb(unsigned long long seed)
+/* src/domain.c:2922-3051: the divergence of (u, v, w) over every cell,
+ * ghosts included, divided by the cell count, into p; each interior face
+ * value of u, v, w (and u0, v0, w0) less the mean of the two cells either
+ * side; then each component less its mean over the faces the reference
+ * averages, over the whole ghosted array */
+int bb_host_turb_correct(void)
 {
   const grid_info *X = &Dom.Gfx, *Y = &Dom.Gfy, *Z = &Dom.Gfz, *G = &Dom.Gcc;
-  memset(p0, 0, sizeof(real) * G->s3b);
-  memset(p, 0, sizeof(real) * G->s3b);
-  memset(phi, 0, sizeof(real) * G->s3b);
-  real *face[3][8] = {
-    {u, diff0_u, diff_u, u0, conv0_u, conv_u, f_x, u_star},
-    {v, diff0_v, diff_v, v0, conv0_v, conv_v, f_y, v_star},
-    {w, diff0_w, diff_w, w0, conv0_w, conv_w, f_z, w_star}};
+  real *face[3][2] = {{u, u0}, {v, v0}, {w, w0}};
   const int n3[3] = {X->s3b, Y->s3b, Z->s3b};
-  for(int c = 0; c < 3; c++)
-    for(int a = 0; a < 8; a++) memset(face[c][a], 0, sizeof(real) * n3[c]);
-
-  rng_init(seed);
-  turbl = (Dom.xl + Dom.yl + Dom.zl) / 3.;
-  const real urms = 3 * turbA * turbl;
-  for(int c = 0; c < 3; c++) {
-    real *x = face[c][0], *x0 = face[c][3];
-    for(int i = 0; i < n3[c]; i++) {
-      const real tmp = (rng_dbl() - 0.5) * urms;
-      x[i] = tmp;
-      x0[i] = tmp;
-    }
-  }
-
   /* the divergence over every cell, ghosts included, over the cell count */
   const real vol = Dom.xn * Dom.yn * Dom.zn;
   for(int k = G->ksb; k < G->keb; k++)
@@ -552,12 +527,52 @@ int bb_host_init_turb(unsigned long long seed)
   wmean /= vol;
   const real mean[3] = {umean, vmean, wmean};
   for(int c = 0; c < 3; c++) {
-    real *x = face[c][0], *x0 = face[c][3];
+    real *x = face[c][0], *x0 = face[c][1];
     for(int i = 0; i < n3[c]; i++) {
       x[i] = x[i] - mean[c];
       x0[i] = x0[i] - mean[c];
     }
   }
+
+  return 0;
+}
+
+/* src/domain.c:2856-3093 after the grids and arrays exist (bb_host_domain):
+ * uniform random velocities of amplitude 3 turbA l (l the mean side
+ * length, the integral scale turbl), the reference's one-sweep divergence
+ * correction and mean removal, zeroed inflow planes, and the first dt
+ * from the viscous limit alone.  The generator is seeded with `seed`
+ * (the reference uses the wall clock).  As there, the correction of v
+ * indexes the cell divergence with v's own strides; values that would lie
+ * past the cell array (zn > yn + 2) are taken as 0, where the reference
+ * reads beyond it. */
+int bb_host_init_turb(unsigned long long seed)
+{
+  const grid_info *X = &Dom.Gfx, *Y = &Dom.Gfy, *Z = &Dom.Gfz, *G = &Dom.Gcc;
+  memset(p0, 0, sizeof(real) * G->s3b);
+  memset(p, 0, sizeof(real) * G->s3b);
+  memset(phi, 0, sizeof(real) * G->s3b);
+  real *face[3][8] = {
+    {u, diff0_u, diff_u, u0, conv0_u, conv_u, f_x, u_star},
+    {v, diff0_v, diff_v, v0, conv0_v, conv_v, f_y, v_star},
+    {w, diff0_w, diff_w, w0, conv0_w, conv_w, f_z, w_star}};
+  const int n3[3] = {X->s3b, Y->s3b, Z->s3b};
+  for(int c = 0; c < 3; c++)
+    for(int a = 0; a < 8; a++) memset(face[c][a], 0, sizeof(real) * n3[c]);
+
+  rng_init(seed);
+  turbl = (Dom.xl + Dom.yl + Dom.zl) / 3.;
+  const real urms = 3 * turbA * turbl;
+  for(int c = 0; c < 3; c++) {
+    real *x = face[c][0], *x0 = face[c][3];
+    for(int i = 0; i < n3[c]; i++) {
+      const real tmp = (rng_dbl() - 0.5) * urms;
+      x[i] = tmp;
+      x0[i] = tmp;
+    }
+  }
+
+  bb_host_turb_correct();
 
   real *planes[15] = {u_WE, u_SN_S, u_SN_N, u_BT_B, u_BT_T, v_WE_W, v_WE_E,
     v_SN, v_BT_B, v_BT_T, w_WE_W, w_WE_E, w_SN_S, w_SN_N, w_BT};

@@ -44,6 +44,9 @@ int bb_host_step(void);
 /* the random initial field of src/domain.c:2856-3093 on the arrays that
  * bb_host_domain made, seeded with `seed`; sets turbl and the first dt */
 int bb_host_init_turb(unsigned long long seed);
+/* its correction and mean removal alone, on the current u, v, w (and u0,
+ * v0, w0), leaving the scaled divergence in p (src/domain.c:2922-3051) */
+int bb_host_turb_correct(void);
 /* src/domain.c:2349 (grids, arrays, the field seeded from the clock);
  * EXIT_SUCCESS or EXIT_FAILURE */
 int domain_init_turb(void);

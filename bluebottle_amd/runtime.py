@@ -98,6 +98,7 @@ HOST_FUNCS = {
     "bb_out_restart_turb": (C.c_int, [C.c_char_p]),
     "bb_in_restart_turb": (C.c_int, [C.c_char_p]),
     "bb_host_init_turb": (C.c_int, [C.c_ulonglong]),
+    "bb_host_turb_correct": (C.c_int, []),
     "bb_host_prepare_turb": (C.c_int, []),
     "bb_host_restart_turb": (C.c_int, [C.c_char_p]),
     "bb_host_step_turb": (C.c_int, []),

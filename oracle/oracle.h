@@ -126,6 +126,8 @@ unsigned long long orc_rng_seeded(unsigned long long state,
 double orc_rng_dbl(unsigned long long *state);
 /* src/domain.c:2856-3093 (field part), drawing from *rng_state */
 void orc_init_turb(orc_state *s, unsigned long long *rng_state);
+/* its correction and mean removal alone (src/domain.c:2922-3051) */
+void orc_turb_correct(orc_state *s);
 /* src/bluebottle.c:945-1016 and 1036-1081: the precursor's set-up after
  * the pushes and one step */
 void orc_prepare_turb(orc_state *s);

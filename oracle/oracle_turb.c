@@ -10,6 +10,7 @@
  * orc_init_turb:          src/domain.c:2856-3093 (the field part),
  *                         with src/rng.c's generator (Ranq1, Numerical
  *                         Recipes 3rd ed. section 7.1.3)
+ * orc_turb_correct:       src/domain.c:2922-3051, its correction sweep
  * The reductions the reference does with thrust::reduce are serial sums
  * here, in index order.
  */
@@ -117,30 +118,12 @@ double orc_rng_dbl(unsigned long long *state)
   return 5.42101086242752217e-20 * (double) ranq1(state);
 }
 
-void orc_init_turb(orc_state *s, unsigned long long *rng_state)
+/* src/domain.c:2922-3051: divergence into p, the correction of the
+ * interior faces, the mean removal */
+void orc_turb_correct(orc_state *s)
 {
   const dom_struct *d = &s->dom;
   const grid_info *X = &d->Gfx, *Y = &d->Gfy, *Z = &d->Gfz, *G = &d->Gcc;
-  memset(s->p0, 0, sizeof(real) * G->s3b);
-  memset(s->p, 0, sizeof(real) * G->s3b);
-  memset(s->phi, 0, sizeof(real) * G->s3b);
-  real *arr[3][8] = {
-    {s->u, s->diff0_u, s->diff_u, s->u0, s->conv0_u, s->conv_u, s->f_x, s->u_star},
-    {s->v, s->diff0_v, s->diff_v, s->v0, s->conv0_v, s->conv_v, s->f_y, s->v_star},
-    {s->w, s->diff0_w, s->diff_w, s->w0, s->conv0_w, s->conv_w, s->f_z, s->w_star}};
-  const int cnt[3] = {X->s3b, Y->s3b, Z->s3b};
-  for(int c = 0; c < 3; c++)
-    for(int a = 0; a < 8; a++) memset(arr[c][a], 0, sizeof(real) * cnt[c]);
-
-  s->turbl = (d->xl + d->yl + d->zl) / 3.;
-  const real urms = 3 * s->turbA * s->turbl;
-  for(int c = 0; c < 3; c++)
-    for(int i = 0; i < cnt[c]; i++) {
-      const real tmp = (orc_rng_dbl(rng_state) - 0.5) * urms;
-      arr[c][0][i] = tmp;
-      arr[c][3][i] = tmp;
-    }
-
   real *p = s->p, *u = s->u, *v = s->v, *w = s->w;
   real *u0 = s->u0, *v0 = s->v0, *w0 = s->w0;
   const real vol = d->xn * d->yn * d->zn;
@@ -204,6 +187,33 @@ void orc_init_turb(orc_state *s, unsigned long long *rng_state)
   for(int i = 0; i < X->s3b; i++) { u[i] = u[i] - umean; u0[i] = u0[i] - umean; }
   for(int i = 0; i < Y->s3b; i++) { v[i] = v[i] - vmean; v0[i] = v0[i] - vmean; }
   for(int i = 0; i < Z->s3b; i++) { w[i] = w[i] - wmean; w0[i] = w0[i] - wmean; }
+}
+
+void orc_init_turb(orc_state *s, unsigned long long *rng_state)
+{
+  const dom_struct *d = &s->dom;
+  const grid_info *X = &d->Gfx, *Y = &d->Gfy, *Z = &d->Gfz, *G = &d->Gcc;
+  memset(s->p0, 0, sizeof(real) * G->s3b);
+  memset(s->p, 0, sizeof(real) * G->s3b);
+  memset(s->phi, 0, sizeof(real) * G->s3b);
+  real *arr[3][8] = {
+    {s->u, s->diff0_u, s->diff_u, s->u0, s->conv0_u, s->conv_u, s->f_x, s->u_star},
+    {s->v, s->diff0_v, s->diff_v, s->v0, s->conv0_v, s->conv_v, s->f_y, s->v_star},
+    {s->w, s->diff0_w, s->diff_w, s->w0, s->conv0_w, s->conv_w, s->f_z, s->w_star}};
+  const int cnt[3] = {X->s3b, Y->s3b, Z->s3b};
+  for(int c = 0; c < 3; c++)
+    for(int a = 0; a < 8; a++) memset(arr[c][a], 0, sizeof(real) * cnt[c]);
+
+  s->turbl = (d->xl + d->yl + d->zl) / 3.;
+  const real urms = 3 * s->turbA * s->turbl;
+  for(int c = 0; c < 3; c++)
+    for(int i = 0; i < cnt[c]; i++) {
+      const real tmp = (orc_rng_dbl(rng_state) - 0.5) * urms;
+      arr[c][0][i] = tmp;
+      arr[c][3][i] = tmp;
+    }
+
+  orc_turb_correct(s);
 
   for(int q = 0; q < 15; q++) {
     size_t n = 0;

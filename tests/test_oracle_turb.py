@@ -450,3 +450,118 @@ def test_turb_config_round_trip_and_reference_file(tmp_path):
     assert c.precursor and c.turbA == 3.0 and c.n == (64, 64, 64)
     assert (c.lo, c.hi) == ((-4.0,) * 3, (4.0,) * 3)
     assert c.out_plane == abi.TOP and c.nu == 0.5
+
+
+# ---------------------------------------------------------------------------
+# the correction sweep of the precursor's initial field, on designed fields
+def _correct_both(case, fill):
+    """Run bb_host_turb_correct and orc_turb_correct on the same fields
+    (fill(s) sets u, v, w on the Simulation; u0 = u etc.); returns the
+    Simulation (open) and the oracle state."""
+    s = sim.Simulation(case)
+    for nm in ("u", "v", "w"):
+        s.array(nm)[:] = 0.0
+    fill(s)
+    for nm in ("u", "v", "w"):
+        s.array(nm + "0")[:] = s.array(nm)
+    o = OracleState.from_sim(s)
+    s.host.bb_host_turb_correct()
+    o.lib.orc_turb_correct(o.ptr)
+    for nm in ("u", "v", "w", "u0", "v0", "w0", "p"):
+        assert np.array_equal(s.array(nm), o.array(nm)), nm
+    return s, o
+
+
+def _interior(shape, lo=(1, 1, 1), hi=None):
+    m = np.zeros(shape, bool)
+    hi = hi or tuple(n - 1 for n in shape)
+    m[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = True
+    return m
+
+
+def test_correction_of_a_uniform_divergence():
+    """u = a x at every face, v = w = 0: the divergence is a in every cell,
+    ghosts included, so p = a / N; the interior faces of all three
+    components lose a / N, and the mean removal (over N faces of each)
+    leaves v and w zero inside and a / N on their ghost faces, and u equal
+    to a (x - the mean of the first xn face positions) inside, that plus
+    a / N on its ghost faces.  Powers of two keep every value exact."""
+    a = 0.5
+    case = turb_case(n=(8, 4, 4), hi=(2.0, 1.0, 1.0))
+    def fill(s):
+        d = s.dom
+        x = d.xs + (np.arange(d.Gfx.inb) - 1) * d.dx
+        s.array("u")[:] = np.broadcast_to(a * x, (d.Gfx.knb, d.Gfx.jnb,
+                                                  d.Gfx.inb)).ravel()
+    s, o = _correct_both(case, fill)
+    try:
+        d = s.dom
+        N = d.xn * d.yn * d.zn
+        pv = a / N
+        assert np.all(s.array("p") == pv)
+        for nm, G, hi in (("v", d.Gfy, None), ("w", d.Gfz, None)):
+            f = s.field3(nm)
+            inner = _interior(f.shape)
+            assert np.all(f[inner] == 0.0), nm
+            assert np.all(f[~inner] == pv), nm
+        x = d.xs + (np.arange(d.Gfx.inb) - 1) * d.dx
+        xm = x[1:d.xn + 1].mean()
+        u = s.field3("u")
+        want = np.broadcast_to(a * (x - xm), u.shape)
+        # u's interior faces: i = 1 .. xn + 1 (the last face too)
+        inner = np.zeros(u.shape, bool)
+        inner[1:-1, 1:-1, 1:d.xn + 2] = True
+        assert np.allclose(u[inner], want[inner], rtol=0, atol=1e-15)
+        assert np.allclose(u[~inner], (want + pv)[~inner], rtol=0, atol=1e-15)
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("n", [(4, 4, 5), (4, 2, 6)])
+def test_correction_of_v_reads_p_with_v_strides(n):
+    """w = b z^2 at every z-face gives a divergence that depends on the cell
+    layer k alone, p_k = 2 b zc_k / N.  The correction of v samples p at
+    v's own flat index, i + j*s1b + k*(s2b + s1b) in the cell array, which
+    is row j + k of layer k, wrapping into the next layer once j + k passes
+    the rows of a layer; past the end of p it takes 0 (zn > yn + 2, the
+    second case).  A correction with the cell strides would subtract p_k
+    there instead, and fail."""
+    b = 0.25
+    nx, ny, nz = n
+    case = turb_case(n=n, hi=(nx * 0.25, ny * 0.25, nz * 0.25))
+
+    def fill(s):
+        d = s.dom
+        z = d.zs + (np.arange(d.Gfz.knb) - 1) * d.dz
+        s.array("w")[:] = np.broadcast_to(
+            (b * z * z)[:, None, None], (d.Gfz.knb, d.Gfz.jnb, d.Gfz.inb)).ravel()
+    s, o = _correct_both(case, fill)
+    try:
+        d = s.dom
+        N = d.xn * d.yn * d.zn
+        gc = d.Gcc
+        zc = d.zs + (np.arange(gc.knb) - 0.5) * d.dz
+        pk = 2 * b * zc / N
+        p3 = s.array("p").reshape(gc.knb, gc.jnb, gc.inb)
+        assert np.allclose(p3, pk[:, None, None], rtol=0, atol=1e-15)
+
+        def p_at(flat):
+            if flat >= gc.s3b:
+                return 0.0
+            return pk[flat // gc.s2b]
+
+        G = d.Gfy
+        corr = np.zeros((G.knb, G.jnb, G.inb))
+        for k in range(G.ks, G.ke):
+            for j in range(G.js, G.je):
+                for i in range(getattr(G, "is"), G.ie):
+                    S = i + (j - 1) * G.s1b + k * G.s2b
+                    Nn = i + j * G.s1b + k * G.s2b
+                    corr[k, j, i] = -0.5 * (p_at(S) + p_at(Nn))
+        mean = corr[1:-1, 1:G.je - 1, 1:-1].sum() / N
+        want = corr - mean
+        assert np.allclose(s.field3("v"), want, rtol=0, atol=1e-15)
+        # with the cell strides the first interior layer would lose p_1
+        assert not np.allclose(want[1, 1:-1, 1:-1], -pk[1] - mean)
+    finally:
+        s.close()
