@@ -146,8 +146,9 @@ void orc_pp_matvec(const orc_state *s, const real *x, real *y);
 void orc_pp_coeffs(const orc_state *s, int C, real vals[13]);
 /* Jacobi-preconditioned BiCGSTAB with CUSP's monitor semantics
  * (src/cuda_bicgstab.cu:1016-1018; CUSP itself is not vendored, so this is
- * the published right-preconditioned algorithm; iterate sequence parity
- * unpinned, converged-solution parity pinned by the residual). */
+ * the published right-preconditioned algorithm; its iterates are pinned
+ * against SciPy's implementation of the same method, its converged
+ * solution by the residual). */
 int orc_bicgstab(const orc_state *s, const real *b, real *x, int max_iter,
   real rel_tol, real *resid_out, int *converged);
 /* The whole entry point: src/cuda_bicgstab.cu:715-1073. */
@@ -200,7 +201,8 @@ void orc_helmholtz_coeffs(const orc_state *s, int comp, real *values);
 void orc_helmholtz_matvec(const orc_state *s, int comp, const real *values,
   const real *x, real *y);
 /* Diagonally preconditioned CG with CUSP's monitor semantics (CUSP is not
- * vendored: iterate-level parity unpinned, solution pinned by residual). */
+ * vendored: the iterates are pinned against SciPy's preconditioned CG, the
+ * solution by its residual). */
 int orc_pcg(const orc_state *s, int comp, const real *values, const real *b,
   real *x, int max_iter, real rel_tol, real *resid_out, int *converged);
 

@@ -22,8 +22,8 @@
  * CUSP is not vendored: the conjugate gradient is the published
  * preconditioned algorithm with CUSP's monitor semantics (stop when the
  * updated ||r|| <= tol * ||b|| at the loop head, or at the iteration cap);
- * iterate-level parity is unpinned, the converged solution is pinned by
- * its residual.
+ * its iterates are pinned against SciPy's preconditioned CG, the converged
+ * solution by its residual.
  */
 #include <math.h>
 #include <stdio.h>
